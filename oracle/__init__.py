@@ -1,0 +1,217 @@
+"""Checkers for the P3 product: test infrastructure, never imported by the product.
+
+* ``run_oracle``: the C restatement of p3_main (libp3oracle.so, built from
+  p3_oracle.c), FP64, OpenMP over columns.
+* ``run_ref`` / ``ref_tables`` / ``ref_lookup_*``: the reference's own Fortran
+  P3 (micro_p3.F90) compiled from /root/reference into _ref/libp3ref.so; only
+  present where the reference sources are (this container).
+
+Arrays are level-major numpy float64 (shape (nlev, ncol); fluxes (nlev+1, ncol)).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "libp3oracle.so")
+ORACLE_FMATH_SO = os.path.join(HERE, "libp3oracle_fmath.so")
+REF_SO = os.path.join(HERE, "_ref", "libp3ref.so")
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+RUNTIME_KEYS = (
+    "max_total_ni", "autoconversion_prefactor", "autoconversion_qc_exponent",
+    "autoconversion_nc_exponent", "autoconversion_radius", "accretion_prefactor",
+    "accretion_qc_exponent", "accretion_qr_exponent", "rain_selfcollection_prefactor",
+    "rain_selfcollection_breakup_diameter", "constant_mu_rain", "spa_ccn_to_nc_factor",
+    "cldliq_to_ice_collection_factor", "rain_to_ice_collection_factor", "min_rime_rho",
+    "max_rime_rho", "immersion_freezing_exponent", "deposition_nucleation_exponent",
+    "ice_sedimentation_factor")
+
+# P3Runtime defaults, components/eamxx/src/physics/p3/p3_functions.hpp:114-133
+RUNTIME_DEFAULTS = dict(
+    max_total_ni=740.0e3, autoconversion_prefactor=1350.0, autoconversion_qc_exponent=2.47,
+    autoconversion_nc_exponent=1.79, autoconversion_radius=25.0e-6, accretion_prefactor=67.0,
+    accretion_qc_exponent=1.15, accretion_qr_exponent=1.15, rain_selfcollection_prefactor=5.78,
+    rain_selfcollection_breakup_diameter=0.00028, constant_mu_rain=1.0,
+    spa_ccn_to_nc_factor=1.0, cldliq_to_ice_collection_factor=0.5,
+    rain_to_ice_collection_factor=1.0, min_rime_rho=50.0, max_rime_rho=900.0,
+    immersion_freezing_exponent=0.65, deposition_nucleation_exponent=0.304,
+    ice_sedimentation_factor=1.0, do_ice_production=True)
+
+
+class _Runtime(C.Structure):
+    _fields_ = [(k, C.c_double) for k in RUNTIME_KEYS] + [("do_ice_production", C.c_int)]
+
+
+class _Tables(C.Structure):
+    _fields_ = [(k, _dp) for k in ("ice_table", "collect_table", "vn_table", "vm_table",
+                                   "revap_table", "mu_r_table", "dnu")]
+
+
+STATE = ("qc", "nc", "qr", "nr", "qi", "qm", "ni", "bm", "qv", "th")
+INPUTS = ("nc_nuceat_tend", "nccn", "ni_activated", "inv_qc_relvar", "cld_frac_i",
+          "cld_frac_l", "cld_frac_r", "pres", "dz", "dpres", "inv_exner", "qv_prev", "t_prev")
+OUTS = ("qv2qi_depos_tend", "precip_liq_surf", "precip_ice_surf", "diag_eff_radius_qc",
+        "diag_eff_radius_qi", "diag_eff_radius_qr", "rho_qi", "precip_liq_flux",
+        "precip_ice_flux", "precip_total_tend", "nevapr")
+HIST = ("liq_ice_exchange", "vap_liq_exchange", "vap_ice_exchange")
+
+
+class _Fields(C.Structure):
+    _fields_ = [(k, _dp) for k in STATE + INPUTS + OUTS + HIST]
+
+
+class _Debug(C.Structure):
+    _fields_ = [("col_status", _ip), ("sed_substeps", _ip), ("ice_table_idx", _ip)]
+
+
+_oracle = {}
+_ref = None
+
+
+def load_oracle(fortran_math=False):
+    """The restatement; ``fortran_math`` selects the build that spells cbrt and
+    expm1 the way the non-BFB Fortran reference does."""
+    so = ORACLE_FMATH_SO if fortran_math else ORACLE_SO
+    if so not in _oracle:
+        if not os.path.exists(so):
+            raise RuntimeError(f"{so} is not built (run `make -C oracle`)")
+        lib = C.CDLL(so)
+        lib.p3o_main.restype = C.c_int
+        lib.p3o_main.argtypes = [C.POINTER(_Runtime), C.POINTER(_Tables), C.POINTER(_Fields),
+                                 C.POINTER(_Debug), C.c_int, C.c_int, C.c_double, C.c_int,
+                                 C.c_int, C.c_int]
+        lib.p3o_qv_sat_dry.restype = C.c_double
+        lib.p3o_qv_sat_dry.argtypes = [C.c_double, C.c_double, C.c_int]
+        lib.p3o_lookup_ice.argtypes = [C.c_double] * 4 + [_ip] * 3 + [_dp] * 3
+        lib.p3o_lookup_rain.argtypes = [C.c_double, C.c_double, _ip, _dp]
+        lib.p3o_apply_table_ice.restype = C.c_double
+        lib.p3o_apply_table_ice.argtypes = [C.POINTER(_Tables), C.c_int] + [C.c_double] * 4
+        lib.p3o_apply_table_coll.restype = C.c_double
+        lib.p3o_apply_table_coll.argtypes = [C.POINTER(_Tables), C.c_int] + [C.c_double] * 6
+        lib.p3o_upwind_step.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_int,
+                                        _dp, _dp, _dp, C.POINTER(_dp), C.POINTER(_dp),
+                                        C.POINTER(_dp)]
+        _oracle[so] = lib
+    return _oracle[so]
+
+
+def _ptr(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+def _runtime(rt):
+    r = dict(RUNTIME_DEFAULTS)
+    r.update(rt or {})
+    s = _Runtime()
+    for k in RUNTIME_KEYS:
+        setattr(s, k, float(r[k]))
+    s.do_ice_production = int(bool(r["do_ice_production"]))
+    return s
+
+
+def _tables(tab):
+    keep = {k: np.ascontiguousarray(tab[k], dtype=np.float64)
+            for k in ("ice", "coll", "vn", "vm", "revap", "mu_r", "dnu")}
+    t = _Tables(_ptr(keep["ice"]), _ptr(keep["coll"]), _ptr(keep["vn"]), _ptr(keep["vm"]),
+                _ptr(keep["revap"]), _ptr(keep["mu_r"]), _ptr(keep["dnu"]))
+    return t, keep
+
+
+def run_oracle(inputs: dict, tables: dict, dt: float, predict_nc=True, prescribed_ccn=False,
+               runtime=None, nthreads=0, fortran_math=False):
+    """p3_main on copies of ``inputs``; returns (fields dict, debug dict, n_errors)."""
+    lib = load_oracle(fortran_math)
+    nlev, ncol = inputs["qc"].shape
+    f = {k: np.array(inputs[k], dtype=np.float64, order="C") for k in STATE + INPUTS}
+    for k in OUTS + HIST:
+        if k in ("precip_liq_surf", "precip_ice_surf"):
+            f[k] = np.zeros(ncol)
+        elif k in ("precip_liq_flux", "precip_ice_flux"):
+            f[k] = np.zeros((nlev + 1, ncol))
+        else:
+            f[k] = np.array(inputs[k]) if k in inputs else np.zeros((nlev, ncol))
+    fs = _Fields(*[_ptr(f[k]) for k in STATE + INPUTS + OUTS + HIST])
+    dbg = dict(col_status=np.zeros(ncol, np.int32), sed_substeps=np.zeros((3, ncol), np.int32),
+               ice_table_idx=np.zeros((nlev, ncol), np.int32))
+    ds = _Debug(*[dbg[k].ctypes.data_as(_ip) for k in ("col_status", "sed_substeps",
+                                                         "ice_table_idx")])
+    rt = _runtime(runtime)
+    tb, keep = _tables(tables)
+    nerr = lib.p3o_main(C.byref(rt), C.byref(tb), C.byref(fs), C.byref(ds), ncol, nlev,
+                        float(dt), int(predict_nc), int(prescribed_ccn), int(nthreads))
+    del keep
+    return f, dbg, nerr
+
+
+# ---------------------------------------------------------------- Fortran reference
+
+def ref_available() -> bool:
+    return os.path.exists(REF_SO)
+
+
+def load_ref(table_dir: str):
+    global _ref
+    if _ref is None:
+        if not ref_available():
+            raise RuntimeError(f"{REF_SO} is not built (run `make -C oracle ref`)")
+        lib = C.CDLL(REF_SO)
+        lib.p3ref_init.restype = C.c_int
+        lib.p3ref_init.argtypes = [C.c_char_p, C.c_int]
+        d = table_dir.encode()
+        info = lib.p3ref_init(d, len(d))
+        if info != 0:
+            raise RuntimeError("p3ref_init failed")
+        lib.p3ref_get_tables.argtypes = [_dp] * 6
+        lib.p3ref_lookup_ice.argtypes = [C.c_double] * 4 + [C.c_int] + [_ip] * 3 + [_dp] * 4
+        lib.p3ref_lookup_coll.argtypes = [C.c_double] * 6 + [C.c_int, _ip, _dp, _dp]
+        _ref = lib
+    return _ref
+
+
+def ref_tables(table_dir: str) -> dict:
+    """The tables exactly as the Fortran reference holds them after p3_init."""
+    lib = load_ref(table_dir)
+    t = dict(ice=np.zeros((5, 4, 50, 12)), coll=np.zeros((5, 4, 50, 30, 2)),
+             vn=np.zeros((300, 10)), vm=np.zeros((300, 10)), revap=np.zeros((300, 10)),
+             mu_r=np.zeros(150))
+    lib.p3ref_get_tables(*[_ptr(t[k]) for k in ("ice", "coll", "vn", "vm", "revap", "mu_r")])
+    # dnu is a constant of micro_p3_utils_init (micro_p3_utils.F90:185-200)
+    t["dnu"] = np.array([0.000, -0.557, -0.430, -0.307, -0.186, -0.067, -0.050, -0.167,
+                         -0.282, -0.397, -0.512, -0.626, -0.739, -0.853, -0.966, -0.966])
+    return t
+
+
+def run_ref(inputs: dict, table_dir: str, dt: float, predict_nc=True, prescribed_ccn=False,
+            it=1):
+    """The reference Fortran p3_main on copies of ``inputs`` (FP64)."""
+    lib = load_ref(table_dir)
+    nlev, ncol = inputs["qc"].shape
+    f = {k: np.array(inputs[k], dtype=np.float64, order="C") for k in STATE + INPUTS}
+    z = lambda: np.zeros((nlev, ncol))
+    for k in ("diag_eff_radius_qc", "diag_eff_radius_qi", "diag_eff_radius_qr", "rho_qi",
+              "qv2qi_depos_tend", "precip_total_tend", "nevapr", "qr_evap_tend", "mu_c",
+              "lamc") + HIST:
+        f[k] = z()
+    f["precip_liq_surf"] = np.zeros(ncol)
+    f["precip_ice_surf"] = np.zeros(ncol)
+    f["precip_liq_flux"] = np.zeros((nlev + 1, ncol))
+    f["precip_ice_flux"] = np.zeros((nlev + 1, ncol))
+    order = ("qc", "nc", "qr", "nr", "th", "qv", "qi", "qm", "ni", "bm", "pres", "dz",
+             "nc_nuceat_tend", "nccn", "ni_activated", "inv_qc_relvar", "precip_liq_surf",
+             "precip_ice_surf", "diag_eff_radius_qc", "diag_eff_radius_qi",
+             "diag_eff_radius_qr", "rho_qi", "dpres", "inv_exner", "qv2qi_depos_tend",
+             "precip_total_tend", "nevapr", "qr_evap_tend", "precip_liq_flux",
+             "precip_ice_flux", "cld_frac_r", "cld_frac_l", "cld_frac_i", "mu_c", "lamc",
+             "liq_ice_exchange", "vap_liq_exchange", "vap_ice_exchange", "qv_prev", "t_prev")
+    lib.p3ref_main.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int] + \
+        [_dp] * len(order)
+    lib.p3ref_main(ncol, nlev, float(dt), int(it), int(predict_nc), int(prescribed_ccn),
+                   *[_ptr(f[k]) for k in order])
+    return f

@@ -1,0 +1,1496 @@
+/*
+ * p3_oracle.c -- CPU restatement of EAMxx P3 microphysics p3_main.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP product, never part of it.
+ *
+ * Restates, scalar (pack size 1), kdir = -1:
+ *   components/eamxx/src/physics/p3/impl/ (P3I below), all *_impl.hpp
+ *   components/eamxx/src/physics/share/physics_saturation_impl.hpp
+ *   components/eamxx/src/physics/share/physics_constants.hpp
+ * Each function cites the reference lines it follows.  Floating-point
+ * expressions keep the reference's operand order (C evaluates left to right
+ * and the build uses -ffp-contract=off), so results follow the reference's
+ * rounding except where libm itself differs.
+ *
+ * One deliberate choice: all nlev+1 entries of precip_liq_flux and
+ * precip_ice_flux are zeroed at the start of a column, as the Fortran
+ * reference does (micro_p3.F90:1311-1312).  The C++ zero-init loop
+ * (p3_main_impl.hpp:50-66) covers nk_pack packs only, so whether the
+ * surface entry is zeroed there depends on the pack size.
+ */
+#include "p3_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ---- constants: physics_constants.hpp:27-113 (same expressions) ---- */
+static const double Cpair = 1004.64;
+static const double Rair = 287.042;
+static const double RH2O = 461.505;
+static const double RV = 461.505;
+static const double RHO_H2O = 1000.0;
+static const double INV_RHO_H2O = 1.0 / 1000.0;
+static const double MWH2O = 18.016;
+static const double MWdry = 28.966;
+#define EP_2 (MWH2O / MWdry)
+static const double gravit = 9.80616;
+static const double LatVap = 2501000.0;
+static const double LatIce = 333700.0;
+static const double CpLiq = 4188.0;
+static const double Tmelt = 273.15;
+#define T_zerodegc Tmelt
+#define T_homogfrz (Tmelt - 40)
+#define T_rainfrz (Tmelt - 4)
+static const double Pi = 3.14159265358979323;
+#define RHOW RHO_H2O
+#define INV_RHO_RIMEMAX (1.0 / 900.0)
+#define THIRD (1.0 / 3.0)
+#define SXTH (1.0 / 6.0)
+#define PIOV3 (Pi * THIRD)
+#define PIOV6 (Pi * SXTH)
+#define BIMM 2.0
+#define CONS1 (PIOV6 * RHOW)
+#define CONS2 (4. * PIOV3 * RHOW)
+#define CONS3 (1.0 / (CONS2 * 1.562500000000000e-14))
+#define CONS5 (PIOV6 * BIMM)
+#define CONS6 (PIOV6 * PIOV6 * RHOW * BIMM)
+static const double QSMALL = 1.e-14;
+static const double QTENDSMALL = 1e-20;
+static const double BSMALL = 1.e-15;
+static const double NSMALL = 1.e-16;
+#define P0 100000.0
+#define RHO_1000MB (P0 / (Rair * Tmelt))
+#define RHO_600MB (60000 / (Rair * 253.15))
+#define INV_CP (1.0 / Cpair)
+static const double dt_left_tol = 1.e-4;
+static const double bcn = 2.;
+static const double dropmass = 5.2e-7;
+static const double NCCNST = 200.0e+6;
+static const double incloud_limit = 5.1e-3;
+static const double precip_limit = 1.0e-2;
+static const double f1r = 0.78;
+static const double f2r = 0.32;
+static const double nmltratio = 1.0;
+/* p3_functions.hpp:40 */
+static const double lookup_table_1a_dum1_c = 4.135985029041767e+00;
+
+/* The reference C++ calls cbrt and expm1.  Its Fortran twin, built without
+ * SCREAM_CONFIG_IS_CMAKE, spells them x**(1/3) and exp(x)-1
+ * (components/eam/src/physics/cam/bfb_math.inc:20-27).  Building with
+ * -DP3O_FORTRAN_MATH selects the Fortran spelling so the restatement can be
+ * compared with the Fortran reference bit for bit. */
+#ifdef P3O_FORTRAN_MATH
+#define P3_CBRT(x) pow((x), 1.0 / 3.0)
+#define P3_EXPM1(x) (exp(x) - 1)
+#else
+#define P3_CBRT(x) cbrt(x)
+#define P3_EXPM1(x) expm1(x)
+#endif
+
+/* ekat::impl::min/max on scalars (ternary form) */
+static inline double dmin(double a, double b) { return a < b ? a : b; }
+static inline double dmax(double a, double b) { return a > b ? a : b; }
+static inline int imin(int a, int b) { return a < b ? a : b; }
+static inline int imax(int a, int b) { return a > b ? a : b; }
+
+/* ================= saturation: physics_saturation_impl.hpp ================= */
+
+/* check_temperature, :14-36 (reports instead of aborting) */
+static inline void check_temperature(double t, int* err) {
+  if (t <= 0 || isnan(t)) *err = 1;
+}
+
+/* MurphyKoop_svp, :38-77 */
+static double murphy_koop_svp(double t, int ice, int* err) {
+  check_temperature(t, err);
+  if (ice && t < Tmelt) {
+    return exp(9.550426 - (5723.265 / t) + (3.53068 * log(t)) - (0.00728332 * t));
+  }
+  const double logt = log(t);
+  return exp(54.842763 - (6763.22 / t) - (4.210 * logt) + (0.000367 * t) +
+             (tanh(0.0415 * (t - 218.8)) *
+              (53.878 - (1331.22 / t) - (9.44523 * logt) + 0.014025 * t)));
+}
+
+/* qv_sat_dry, :127-157 (MurphyKoop) */
+static double qv_sat_dry_e(double t, double p, int ice, int* err) {
+  const double e = murphy_koop_svp(t, ice, err);
+  return EP_2 * e / dmax(p, 1.e-3);
+}
+
+double p3o_qv_sat_dry(double t, double p, int ice) {
+  int err = 0;
+  return qv_sat_dry_e(t, p, ice, &err);
+}
+
+/* ======================== tables ======================== */
+
+#define ICE_T(tb, jj, ii, i, j) ((tb)->ice_table[((((jj) * 4 + (ii)) * 50 + (i)) * 12) + (j)])
+#define COL_T(tb, jj, ii, i, j, k) \
+  ((tb)->collect_table[(((((jj) * 4 + (ii)) * 50 + (i)) * 30 + (j)) * 2) + (k)])
+
+typedef struct { int dumi, dumii, dumjj; double dum1, dum4, dum5; } table_ice_t;
+typedef struct { int dumj; double dum3; } table_rain_t;
+typedef struct { int dumii, dumjj; double rdumii, rdumjj; } table3_t;
+
+/* lookup_ice, P3I/p3_table_ice_impl.hpp:80-138 */
+static void lookup_ice(double qi, double ni, double qm, double rhop, table_ice_t* t) {
+  t->dum1 = (log10(qi / ni) + 18) * lookup_table_1a_dum1_c - 10;
+  t->dumi = (int)t->dum1;
+  t->dum1 = dmin(t->dum1, 50.0);
+  t->dum1 = dmax(t->dum1, 1.);
+  t->dumi = imax(1, t->dumi);
+  t->dumi = imin(50 - 1, t->dumi);
+  t->dum4 = (qm / qi) * 3 + 1;
+  t->dumii = (int)t->dum4;
+  t->dum4 = dmin(t->dum4, 4.0);
+  t->dum4 = dmax(t->dum4, 1.);
+  t->dumii = imax(1, t->dumii);
+  t->dumii = imin(4 - 1, t->dumii);
+  if (rhop <= 650) t->dum5 = (rhop - 50) * 0.005 + 1;
+  else t->dum5 = (rhop - 650) * 0.004 + 4;
+  t->dumjj = (int)t->dum5;
+  t->dum5 = dmin(t->dum5, 5.0);
+  t->dum5 = dmax(t->dum5, 1.);
+  t->dumjj = imax(1, t->dumjj);
+  t->dumjj = imin(5 - 1, t->dumjj);
+  t->dumi -= 1;
+  t->dumjj -= 1;
+  t->dumii -= 1;
+}
+
+/* lookup_rain, P3I/p3_table_ice_impl.hpp:140-173 */
+static void lookup_rain(double qr, double nr, table_rain_t* t) {
+  const int gt_small = qr > QSMALL && nr > 0.0;
+  double dumlr = 1;
+  if (gt_small) dumlr = P3_CBRT(qr / (Pi * RHO_H2O * nr));
+  t->dum3 = (log10(1 * dumlr) + 5) * 10.70415;
+  t->dumj = (int)t->dum3;
+  t->dum3 = dmin(t->dum3, 30.0);
+  t->dum3 = dmax(t->dum3, 1);
+  t->dumj = imax(1, t->dumj);
+  t->dumj = imin(30 - 1, t->dumj);
+  if (!gt_small) {
+    t->dumj = 1;
+    t->dum3 = 1;
+  }
+  t->dumj -= 1;
+}
+
+/* apply_table_ice, P3I/p3_table_ice_impl.hpp:175-217 */
+static double apply_table_ice(const p3o_tables* tb, int idx, const table_ice_t* t) {
+  const int jj = t->dumjj, ii = t->dumii, i = t->dumi;
+  const double w1 = (t->dum1 - (double)i - 1);
+  const double w4 = (t->dum4 - (double)ii - 1);
+  double iproc1 = ICE_T(tb, jj, ii, i, idx) + w1 * (ICE_T(tb, jj, ii, i + 1, idx) - ICE_T(tb, jj, ii, i, idx));
+  double gproc1 = ICE_T(tb, jj, ii + 1, i, idx) + w1 * (ICE_T(tb, jj, ii + 1, i + 1, idx) - ICE_T(tb, jj, ii + 1, i, idx));
+  const double tmp1 = iproc1 + w4 * (gproc1 - iproc1);
+  iproc1 = ICE_T(tb, jj + 1, ii, i, idx) + w1 * (ICE_T(tb, jj + 1, ii, i + 1, idx) - ICE_T(tb, jj + 1, ii, i, idx));
+  gproc1 = ICE_T(tb, jj + 1, ii + 1, i, idx) + w1 * (ICE_T(tb, jj + 1, ii + 1, i + 1, idx) - ICE_T(tb, jj + 1, ii + 1, i, idx));
+  const double tmp2 = iproc1 + w4 * (gproc1 - iproc1);
+  return tmp1 + (t->dum5 - (double)jj - 1) * (tmp2 - tmp1);
+}
+
+/* apply_table_coll, P3I/p3_table_ice_impl.hpp:219-297 */
+static double apply_table_coll(const p3o_tables* tb, int idx, const table_ice_t* ti, const table_rain_t* tr) {
+  const int jj = ti->dumjj, ii = ti->dumii, i = ti->dumi, j = tr->dumj;
+  const double w1 = (ti->dum1 - (double)i - 1);
+  const double w3 = (tr->dum3 - (double)j - 1);
+  const double w4 = (ti->dum4 - (double)ii - 1);
+  double dproc1 = COL_T(tb, jj, ii, i, j, idx) + w1 * (COL_T(tb, jj, ii, i + 1, j, idx) - COL_T(tb, jj, ii, i, j, idx));
+  double dproc2 = COL_T(tb, jj, ii, i, j + 1, idx) + w1 * (COL_T(tb, jj, ii, i + 1, j + 1, idx) - COL_T(tb, jj, ii, i, j + 1, idx));
+  double iproc1 = dproc1 + w3 * (dproc2 - dproc1);
+  dproc1 = COL_T(tb, jj, ii + 1, i, j, idx) + w1 * (COL_T(tb, jj, ii + 1, i + 1, j, idx) - COL_T(tb, jj, ii + 1, i, j, idx));
+  dproc2 = COL_T(tb, jj, ii + 1, i, j + 1, idx) + w1 * (COL_T(tb, jj, ii + 1, i + 1, j + 1, idx) - COL_T(tb, jj, ii + 1, i, j + 1, idx));
+  double gproc1 = dproc1 + w3 * (dproc2 - dproc1);
+  const double tmp1 = iproc1 + w4 * (gproc1 - iproc1);
+  dproc1 = COL_T(tb, jj + 1, ii, i, j, idx) + w1 * (COL_T(tb, jj + 1, ii, i + 1, j, idx) - COL_T(tb, jj + 1, ii, i, j, idx));
+  dproc2 = COL_T(tb, jj + 1, ii, i, j + 1, idx) + w1 * (COL_T(tb, jj + 1, ii, i + 1, j + 1, idx) - COL_T(tb, jj + 1, ii, i, j + 1, idx));
+  iproc1 = dproc1 + w3 * (dproc2 - dproc1);
+  dproc1 = COL_T(tb, jj + 1, ii + 1, i, j, idx) + w1 * (COL_T(tb, jj + 1, ii + 1, i + 1, j, idx) - COL_T(tb, jj + 1, ii + 1, i, j, idx));
+  dproc2 = COL_T(tb, jj + 1, ii + 1, i, j + 1, idx) + w1 * (COL_T(tb, jj + 1, ii + 1, i + 1, j + 1, idx) - COL_T(tb, jj + 1, ii + 1, i, j + 1, idx));
+  gproc1 = dproc1 + w3 * (dproc2 - dproc1);
+  const double tmp2 = iproc1 + w4 * (gproc1 - iproc1);
+  return tmp1 + (ti->dum5 - (double)jj - 1) * (tmp2 - tmp1);
+}
+
+/* lookup (Table3), P3I/p3_table3_impl.hpp:14-62 */
+static void lookup3(double mu_r, double lamr, table3_t* t) {
+  const double dum1 = (mu_r + 1) / lamr;
+  if (dum1 <= 195.e-6) {
+    const double inv_dum3 = 0.1;
+    double rdumii = (dum1 * 1.e6 + 5) * inv_dum3;
+    rdumii = dmax(rdumii, 1.0);
+    rdumii = dmin(rdumii, 20.0);
+    int dumii = (int)rdumii;
+    dumii = imax(dumii, 1);
+    dumii = imin(dumii, 20);
+    t->rdumii = rdumii;
+    t->dumii = dumii;
+  } else {
+    const double inv_dum3 = THIRD * 0.1;
+    double rdumii = (dum1 * 1.e+6 - 195) * inv_dum3 + 20;
+    rdumii = dmax(rdumii, 20.0);
+    rdumii = dmin(rdumii, 300.0);
+    int dumii = (int)rdumii;
+    dumii = imax(dumii, 20);
+    dumii = imin(dumii, 299);
+    t->rdumii = rdumii;
+    t->dumii = dumii;
+  }
+  double rdumjj = mu_r + 1;
+  rdumjj = dmax(rdumjj, 1);
+  rdumjj = dmin(rdumjj, 10);
+  int dumjj = (int)rdumjj;
+  dumjj = imax(dumjj, 1);
+  dumjj = imin(dumjj, 9);
+  t->rdumjj = rdumjj;
+  t->dumjj = dumjj;
+}
+
+/* apply_table, P3I/p3_table3_impl.hpp:64-83; table is [300][10] */
+static double apply_table3(const double* table, const table3_t* t) {
+  const double rdumii_m_dumii = t->rdumii - (double)t->dumii;
+  const double t_im1_jm1 = table[(t->dumii - 1) * 10 + (t->dumjj - 1)];
+  const double dum1 = (t_im1_jm1 + rdumii_m_dumii * (table[t->dumii * 10 + (t->dumjj - 1)] - t_im1_jm1));
+  const double t_im1_j = table[(t->dumii - 1) * 10 + t->dumjj];
+  const double dum2 = (t_im1_j + rdumii_m_dumii * (table[t->dumii * 10 + t->dumjj] - t_im1_j));
+  return dum1 + (t->rdumjj - (double)t->dumjj) * (dum2 - dum1);
+}
+
+/* exported wrappers for unit tests */
+void p3o_lookup_ice(double qi, double ni, double qm, double rhop, int* dumi, int* dumii,
+                    int* dumjj, double* dum1, double* dum4, double* dum5) {
+  table_ice_t t;
+  lookup_ice(qi, ni, qm, rhop, &t);
+  *dumi = t.dumi; *dumii = t.dumii; *dumjj = t.dumjj;
+  *dum1 = t.dum1; *dum4 = t.dum4; *dum5 = t.dum5;
+}
+void p3o_lookup_rain(double qr, double nr, int* dumj, double* dum3) {
+  table_rain_t t;
+  lookup_rain(qr, nr, &t);
+  *dumj = t.dumj; *dum3 = t.dum3;
+}
+double p3o_apply_table_ice(const p3o_tables* tab, int idx, double qi, double ni, double qm, double rhop) {
+  table_ice_t t;
+  lookup_ice(qi, ni, qm, rhop, &t);
+  return apply_table_ice(tab, idx, &t);
+}
+double p3o_apply_table_coll(const p3o_tables* tab, int idx, double qi, double ni, double qm,
+                            double rhop, double qr, double nr) {
+  table_ice_t ti;
+  table_rain_t tr;
+  lookup_ice(qi, ni, qm, rhop, &ti);
+  lookup_rain(qr, nr, &tr);
+  return apply_table_coll(tab, idx, &ti, &tr);
+}
+
+/* ======================== size distributions ======================== */
+
+/* get_cloud_dsd2, P3I/p3_dsd2_impl.hpp:14-71 (iparam = 3: nu stays 0) */
+static void get_cloud_dsd2(double qc, double* nc, double* mu_c, double rho, double* nu,
+                           double* lamc, double* cdist, double* cdist1) {
+  *lamc = 0; *cdist = 0; *cdist1 = 0; *nu = 0; *mu_c = 0;
+  if (qc >= QSMALL) {
+    *nc = dmax(*nc, NSMALL);
+    double m = 0.0005714 * (*nc * 1.e-6 * rho) + 0.2714;
+    m = 1 / (m * m) - 1;
+    m = dmax(m, 2);
+    m = dmin(m, 15);
+    *mu_c = m;
+    *lamc = P3_CBRT(CONS1 * *nc * (m + 3) * (m + 2) * (m + 1) / qc);
+    const double lammin = (m + 1) * 2.5e+4;
+    const double lammax = (m + 1) * 1.e+6;
+    const int lt = *lamc < lammin, gt = *lamc > lammax;
+    if (lt) *lamc = lammin;
+    if (gt) *lamc = lammax;
+    if (lt || gt)
+      *nc = 6 * (*lamc * *lamc * *lamc) * qc / (Pi * RHO_H2O * (m + 3) * (m + 2) * (m + 1));
+    *cdist = *nc * (m + 1) / *lamc;
+    *cdist1 = *nc / tgamma(m + 1);
+  }
+}
+
+/* get_rain_dsd2, P3I/p3_dsd2_impl.hpp:73-126 */
+static void get_rain_dsd2(const p3o_runtime* rt, double qr, double* nr, double* mu_r, double* lamr) {
+  *lamr = 0;
+  if (qr >= QSMALL) {
+    const double nr_lim = dmax(*nr, NSMALL);
+    *mu_r = rt->constant_mu_rain;
+    const double f = CONS1 * (*mu_r + 3) * (*mu_r + 2) * (*mu_r + 1);
+    *lamr = P3_CBRT(f * nr_lim / qr);
+    const double lammax = (*mu_r + 1.) * 1.e+5;
+    const double lammin = (*mu_r + 1.) * 500;
+    const int lt = *lamr < lammin, gt = *lamr > lammax;
+    *nr = nr_lim;
+    if (lt || gt) {
+      if (lt) *lamr = lammin;
+      if (gt) *lamr = lammax;
+      *nr = *lamr * *lamr * *lamr * qr / f;
+    }
+  }
+}
+
+/* get_cdistr_logn0r, P3I/p3_dsd2_impl.hpp:128-148 */
+static void get_cdistr_logn0r(double qr, double nr, double mu_r, double lamr, double* cdistr, double* logn0r) {
+  *cdistr = 0;
+  *logn0r = 0;
+  if (qr >= QSMALL) {
+    *cdistr = nr / tgamma(mu_r + 1);
+    *logn0r = log10(*cdistr) + (mu_r + 1) * log10(lamr);
+  }
+}
+
+/* calc_bulk_rho_rime, P3I/p3_ice_sed_impl.hpp:14-64 */
+static double calc_bulk_rho_rime(const p3o_runtime* rt, double qi_tot, double* qi_rim, double* bi_rim) {
+  double rho_rime = 0;
+  const int gt = *bi_rim >= BSMALL, lt = *bi_rim < BSMALL;
+  if (gt) rho_rime = *qi_rim / *bi_rim;
+  const int lt_min = rho_rime < rt->min_rime_rho, gt_max = rho_rime > rt->max_rime_rho;
+  if (gt && lt_min) rho_rime = rt->min_rime_rho;
+  if (gt && gt_max) rho_rime = rt->max_rime_rho;
+  if (gt && (gt_max || lt_min)) *bi_rim = *qi_rim / rho_rime;
+  if (lt) { *qi_rim = 0; *bi_rim = 0; rho_rime = 0; }
+  if (*qi_rim > qi_tot && rho_rime > 0) {
+    *qi_rim = qi_tot;
+    *bi_rim = *qi_rim / rho_rime;
+  }
+  if (*qi_rim < QSMALL) { *qi_rim = 0; *bi_rim = 0; }
+  return rho_rime;
+}
+
+/* calculate_incloud_mixingratios, P3I/p3_incloud_mixingratios_impl.hpp:9-67 */
+static void incloud_mixingratios(double qc, double qr, double qi, double qm, double nc, double nr,
+                                 double ni, double bm, double icfl, double icfi, double icfr,
+                                 double* qc_i, double* qr_i, double* qi_i, double* qm_i,
+                                 double* nc_i, double* nr_i, double* ni_i, double* bm_i) {
+  if (qc >= QSMALL) { *qc_i = qc * icfl; *nc_i = dmax(nc * icfl, 0); }
+  else { *qc_i = 0; *nc_i = 0; }
+  if (qi >= QSMALL) { *qi_i = qi * icfi; *ni_i = dmax(ni * icfi, 0); }
+  else { *qi_i = 0; *ni_i = 0; }
+  if (qi >= QSMALL && qm >= QSMALL) { *qm_i = qm * icfi; *bm_i = dmax(bm * icfl, 0); }
+  else { *qm_i = 0; *bm_i = 0; }
+  if (qr >= QSMALL) { *qr_i = qr * icfr; *nr_i = dmax(nr * icfr, 0); }
+  else { *qr_i = 0; *nr_i = 0; }
+  if (*qc_i > incloud_limit || *qi_i > incloud_limit || *qr_i > precip_limit || *bm_i > incloud_limit) {
+    *qc_i = dmin(*qc_i, incloud_limit);
+    *qi_i = dmin(*qi_i, incloud_limit);
+    *bm_i = dmin(*bm_i, incloud_limit);
+    *qr_i = dmin(*qr_i, precip_limit);
+  }
+}
+
+/* impose_max_total_ni, P3I/p3_impose_max_total_ni_impl.hpp:9-26 */
+static inline void impose_max_total_ni(double* ni, double max_total_ni, double inv_rho) {
+  if (*ni >= 1e-20) *ni = *ni * dmin(max_total_ni * inv_rho / *ni, 1);
+}
+
+/* ======================== sedimentation ======================== */
+
+/* calc_first_order_upwind_step<kdir=-1,n>, P3I/p3_upwind_impl.hpp:14-91 */
+void p3o_upwind_step(int nk, int k_bot, int k_top, double dt_sub, int nfield,
+                     const double* rho, const double* inv_rho, const double* inv_dz,
+                     double* const* flux, const double* const* V, double* const* r) {
+  (void)nk;
+  for (int k = k_top; k <= k_bot; ++k)
+    for (int f = 0; f < nfield; ++f) flux[f][k] = V[f][k] * r[f][k] * rho[k];
+  {
+    const int k = k_top;
+    for (int f = 0; f < nfield; ++f) {
+      const double fluxdiv = (0 - flux[f][k]) * inv_dz[k];
+      r[f][k] += fluxdiv * dt_sub * inv_rho[k];
+    }
+  }
+  for (int k = k_top + 1; k <= k_bot; ++k)
+    for (int f = 0; f < nfield; ++f) {
+      const double fluxdiv = (flux[f][k - 1] - flux[f][k]) * inv_dz[k];
+      r[f][k] += fluxdiv * dt_sub * inv_rho[k];
+    }
+}
+
+/* generalized_sedimentation<n>, P3I/p3_upwind_impl.hpp:93-129 (kdir = -1) */
+static void generalized_sedimentation(int nk, int nfield, const double* rho, const double* inv_rho,
+                                      const double* inv_dz, int k_qxtop, int* k_qxbot, int kbot,
+                                      double Co_max, double* dt_left, double* prt_accum,
+                                      double* const* flux, const double* const* V, double* const* r) {
+  const int tmpint1 = (int)(Co_max + 1);
+  const double dt_sub = *dt_left / tmpint1;
+  const int k_temp = (*k_qxbot == kbot) ? *k_qxbot : *k_qxbot + 1;
+  p3o_upwind_step(nk, k_temp, k_qxtop, dt_sub, nfield, rho, inv_rho, inv_dz, flux, V, r);
+  if (*k_qxbot == kbot) *prt_accum += flux[0][kbot] * dt_sub;
+  else *k_qxbot += 1;
+  *dt_left -= dt_sub;
+}
+
+/* find_top / find_bottom, P3I/p3_find_impl.hpp:14-92 (serial branch, kdir = -1) */
+static int find_top(const double* v, double small, int nk, int* present) {
+  *present = 0;
+  for (int k = 0; k < nk; ++k)
+    if (v[k] >= small) { *present = 1; return k; }
+  return 0;
+}
+static int find_bottom(const double* v, double small, int kbot, int ktop, int* present) {
+  *present = 0;
+  for (int k = kbot; k >= ktop; --k)
+    if (v[k] >= small) { *present = 1; return k; }
+  return 0;
+}
+
+/* Per-column work arrays (the reference's WorkspaceManager slots). */
+typedef struct col_ws {
+  double *T_atm, *rho, *inv_rho, *qv_sat_l, *qv_sat_i, *qv_supersat_i, *rhofacr, *rhofaci, *acn;
+  double *exner, *inv_dz, *icfl, *icfi, *icfr;
+  double *qc_incld, *qr_incld, *qi_incld, *qm_incld, *nc_incld, *nr_incld, *ni_incld, *bm_incld;
+  double *mu_c, *nu, *lamc, *cdist, *cdist1, *cdistr, *mu_r, *lamr, *logn0r;
+  double *ze_ice, *ze_rain, *diag_vm_qi, *diag_diam_qi, *diag_equiv_reflectivity;
+  double *V1, *V2, *F1, *F2, *F3, *F4;
+  double* mem;
+} col_ws;
+
+static const int NWS = 43;
+
+static void ws_alloc(col_ws* w, int nk) {
+  w->mem = (double*)calloc((size_t)NWS * (nk + 1), sizeof(double));
+  double** p[] = {&w->T_atm, &w->rho, &w->inv_rho, &w->qv_sat_l, &w->qv_sat_i, &w->qv_supersat_i,
+                  &w->rhofacr, &w->rhofaci, &w->acn, &w->exner, &w->inv_dz, &w->icfl, &w->icfi,
+                  &w->icfr, &w->qc_incld, &w->qr_incld, &w->qi_incld, &w->qm_incld, &w->nc_incld,
+                  &w->nr_incld, &w->ni_incld, &w->bm_incld, &w->mu_c, &w->nu, &w->lamc, &w->cdist,
+                  &w->cdist1, &w->cdistr, &w->mu_r, &w->lamr, &w->logn0r, &w->ze_ice, &w->ze_rain,
+                  &w->diag_vm_qi, &w->diag_diam_qi, &w->diag_equiv_reflectivity, &w->V1, &w->V2,
+                  &w->F1, &w->F2, &w->F3, &w->F4};
+  const int n = (int)(sizeof(p) / sizeof(p[0]));
+  for (int j = 0; j < n; ++j) *p[j] = w->mem + (size_t)j * (nk + 1);
+}
+
+/* One column in contiguous local arrays. */
+typedef struct col_io {
+  double *qc, *nc, *qr, *nr, *qi, *qm, *ni, *bm, *qv, *th;
+  double *nc_nuceat_tend, *nccn, *ni_activated, *inv_qc_relvar, *cld_frac_i, *cld_frac_l, *cld_frac_r;
+  double *pres, *dz, *dpres, *inv_exner, *qv_prev, *t_prev;
+  double *qv2qi_depos_tend, *diag_eff_radius_qc, *diag_eff_radius_qi, *diag_eff_radius_qr;
+  double *rho_qi, *precip_liq_flux, *precip_ice_flux, *precip_total_tend, *nevapr;
+  double *liq_ice_exchange, *vap_liq_exchange, *vap_ice_exchange;
+  double precip_liq_surf, precip_ice_surf;
+  int status, nsub[3];
+  int* ice_idx;
+  double* mem;
+} col_io;
+
+/* ========================= p3_main on one column ========================= */
+
+static void p3_column(const p3o_runtime* rt, const p3o_tables* tb, int nk, double dt,
+                      int predictNc, int prescribedCCN, col_io* c, col_ws* w, int* err) {
+  const double inv_dt = 1 / dt;
+  const int kbot = nk - 1; /* ktop = 0 */
+  const int do_ice_production = rt->do_ice_production;
+  const double latvap = LatVap, latice = LatIce, inv_cp = INV_CP, g = gravit;
+
+  memset(w->mem, 0, sizeof(double) * (size_t)NWS * (nk + 1));
+
+  /* ---- p3_main_init, P3I/p3_main_impl.hpp:18-72 ---- */
+  c->precip_liq_surf = 0;
+  c->precip_ice_surf = 0;
+  for (int k = 0; k < nk; ++k) {
+    w->diag_equiv_reflectivity[k] = -99;
+    w->ze_ice[k] = 1.e-22;
+    w->ze_rain[k] = 1.e-22;
+    c->diag_eff_radius_qc[k] = 10.e-6;
+    c->diag_eff_radius_qi[k] = 25.e-6;
+    c->diag_eff_radius_qr[k] = 500.e-6;
+    w->icfi[k] = 1 / c->cld_frac_i[k];
+    w->icfl[k] = 1 / c->cld_frac_l[k];
+    w->icfr[k] = 1 / c->cld_frac_r[k];
+    w->exner[k] = 1 / c->inv_exner[k];
+    w->T_atm[k] = c->th[k] * w->exner[k];
+    c->qv[k] = dmax(c->qv[k], 0);
+    w->inv_dz[k] = 1 / c->dz[k];
+    c->rho_qi[k] = 0;
+    c->qv2qi_depos_tend[k] = 0;
+    c->precip_total_tend[k] = 0;
+    c->nevapr[k] = 0;
+  }
+  for (int k = 0; k <= nk; ++k) { c->precip_liq_flux[k] = 0; c->precip_ice_flux[k] = 0; }
+
+  /* ---- p3_main_part1, P3I/p3_main_impl_part1.hpp:18-185 ---- */
+  int nucleationPossible = 0, hydrometeorsPresent = 0;
+  for (int k = 0; k < nk; ++k) {
+    w->rho[k] = c->dpres[k] / c->dz[k] / g;
+    w->inv_rho[k] = 1 / w->rho[k];
+    w->qv_sat_l[k] = qv_sat_dry_e(w->T_atm[k], c->pres[k], 0, err);
+    w->qv_sat_i[k] = qv_sat_dry_e(w->T_atm[k], c->pres[k], 1, err);
+    w->qv_supersat_i[k] = c->qv[k] / w->qv_sat_i[k] - 1;
+    w->rhofacr[k] = pow(RHO_1000MB * w->inv_rho[k], .54);
+    w->rhofaci[k] = pow(RHO_600MB * w->inv_rho[k], .54);
+    const double dum = 1.496e-6 * pow(w->T_atm[k], 1.5) / (w->T_atm[k] + 120);
+    w->acn[k] = g * RHO_H2O / (18 * dum);
+    if (w->T_atm[k] < T_zerodegc && w->qv_supersat_i[k] >= -0.05) nucleationPossible = 1;
+
+    if (c->qc[k] < QSMALL) {
+      c->qv[k] = c->qv[k] + c->qc[k];
+      c->th[k] = c->th[k] - c->inv_exner[k] * c->qc[k] * latvap * inv_cp;
+      c->qc[k] = 0;
+      c->nc[k] = 0;
+    } else {
+      hydrometeorsPresent = 1;
+      if (prescribedCCN)
+        c->nc[k] = dmax(c->nc[k], rt->spa_ccn_to_nc_factor * c->nccn[k] / w->icfl[k]);
+      else if (predictNc)
+        c->nc[k] = dmax(c->nc[k] + c->nc_nuceat_tend[k] * dt, 0.0);
+      else
+        c->nc[k] = NCCNST * w->inv_rho[k];
+    }
+    if (c->qr[k] < QSMALL) {
+      c->qv[k] = c->qv[k] + c->qr[k];
+      c->th[k] = c->th[k] - c->inv_exner[k] * c->qr[k] * latvap * inv_cp;
+      c->qr[k] = 0;
+      c->nr[k] = 0;
+    } else {
+      hydrometeorsPresent = 1;
+    }
+    if (c->qi[k] < QSMALL || (c->qi[k] < 1.e-8 && w->qv_supersat_i[k] < -0.1)) {
+      c->qv[k] = c->qv[k] + c->qi[k];
+      c->th[k] = c->th[k] - c->inv_exner[k] * c->qi[k] * (latvap + latice) * inv_cp;
+      c->qi[k] = 0; c->ni[k] = 0; c->qm[k] = 0; c->bm[k] = 0;
+    } else {
+      hydrometeorsPresent = 1;
+    }
+    if (c->qi[k] >= QSMALL && c->qi[k] < 1.e-8 && w->T_atm[k] >= T_zerodegc) {
+      c->qr[k] = c->qr[k] + c->qi[k];
+      c->th[k] = c->th[k] - c->inv_exner[k] * c->qi[k] * latice * inv_cp;
+      c->qi[k] = 0; c->ni[k] = 0; c->qm[k] = 0; c->bm[k] = 0;
+    }
+    w->T_atm[k] = c->th[k] * w->exner[k];
+    incloud_mixingratios(c->qc[k], c->qr[k], c->qi[k], c->qm[k], c->nc[k], c->nr[k], c->ni[k], c->bm[k],
+                         w->icfl[k], w->icfi[k], w->icfr[k], &w->qc_incld[k], &w->qr_incld[k],
+                         &w->qi_incld[k], &w->qm_incld[k], &w->nc_incld[k], &w->nr_incld[k],
+                         &w->ni_incld[k], &w->bm_incld[k]);
+  }
+
+  c->status = 0;
+  c->nsub[0] = c->nsub[1] = c->nsub[2] = 0;
+  if (!(nucleationPossible || hydrometeorsPresent)) return;
+
+  /* ---- p3_main_part2, P3I/p3_main_impl_part2.hpp:18-526 ---- */
+  hydrometeorsPresent = 0;
+  for (int k = 0; k < nk; ++k) {
+    const double T_atm = w->T_atm[k], rho = w->rho[k], inv_rho = w->inv_rho[k];
+    const double pres = c->pres[k];
+    if (c->qc[k] < QSMALL && c->qr[k] < QSMALL && c->qi[k] < QSMALL && T_atm < T_zerodegc &&
+        w->qv_supersat_i[k] < -0.05)
+      continue;
+
+    double qc2qr_accret_tend = 0, qr2qv_evap_tend = 0, qc2qr_autoconv_tend = 0, nc_accret_tend = 0,
+           nc_selfcollect_tend = 0, nc2nr_autoconv_tend = 0, nr_selfcollect_tend = 0,
+           nr_evap_tend = 0, ncautr = 0;
+    double qi2qv_sublim_tend = 0, nr_ice_shed_tend = 0, qc2qi_hetero_freeze_tend = 0,
+           qr2qi_collect_tend = 0, qc2qr_ice_shed_tend = 0, qi2qr_melt_tend = 0,
+           qc2qi_collect_tend = 0, qr2qi_immers_freeze_tend = 0, qv2qi_nucleat_tend = 0,
+           ni2nr_melt_tend = 0, nc_collect_tend = 0, ncshdc = 0, nc2ni_immers_freeze_tend = 0,
+           nr_collect_tend = 0, ni_selfcollect_tend = 0, ni_nucleat_tend = 0,
+           qv2qi_vapdep_tend = 0, qc2qi_berg_tend = 0, nr2ni_immers_freeze_tend = 0,
+           ni_sublim_tend = 0, qc_growth_rate = 0;
+    double mu = 0, dv = 0, sc = 0, dqsdt = 0, dqsidt = 0, ab = 0, abi = 0, kap = 0, eii = 0;
+    double table_val_qi_fallspd = 0, table_val_ni_self_collect = 0, table_val_qc2qi_collect = 0,
+           table_val_qi2qr_melting = 0, table_val_nr_collect = 0, table_val_qr2qi_collect = 0,
+           table_val_ni_lammax = 0, table_val_ni_lammin = 0, table_val_qi2qr_vent_melt = 0;
+    double vtrmi1 = 0, rho_qm_cloud = 400, epsi = 0, epsr = 0, epsc = 0, epsi_tot = 0;
+    int wetgrowth = 0;
+    (void)vtrmi1; (void)epsc; (void)dqsidt;
+
+    double* qc_incld = &w->qc_incld[k];
+    double* qr_incld = &w->qr_incld[k];
+    double* qi_incld = &w->qi_incld[k];
+    double* qm_incld = &w->qm_incld[k];
+    double* nc_incld = &w->nc_incld[k];
+    double* nr_incld = &w->nr_incld[k];
+    double* ni_incld = &w->ni_incld[k];
+    double* bm_incld = &w->bm_incld[k];
+    const double cld_frac_l = c->cld_frac_l[k], cld_frac_r = c->cld_frac_r[k], cld_frac_i = c->cld_frac_i[k];
+
+    const int skip_micro = !(*qc_incld >= QSMALL || *qr_incld >= QSMALL || *qi_incld >= QSMALL);
+    if (!skip_micro) {
+      /* get_time_space_phys_variables, P3I/p3_get_time_space_phys_variables_impl.hpp:10-46 */
+      mu = 1.496e-6 * pow(T_atm, 1.5) / (T_atm + 120);
+      dv = 8.794e-5 * pow(T_atm, 1.81) / pres;
+      sc = mu / (rho * dv);
+      {
+        const double dum = 1 / (RV * (T_atm * T_atm));
+        dqsdt = latvap * w->qv_sat_l[k] * dum;
+        dqsidt = (latvap + latice) * w->qv_sat_i[k] * dum;
+        ab = 1 + dqsdt * latvap * INV_CP;
+        abi = 1 + dqsidt * (latvap + latice) * INV_CP;
+        kap = 1.414e+3 * mu;
+        if (T_atm < 253.15) eii = 0.001;
+        else if (T_atm < 273.15) eii = 0.001 + (T_atm - 253.15) * (0.3 - 0.001) / 20;
+        else eii = 0.3;
+      }
+
+      get_cloud_dsd2(*qc_incld, nc_incld, &w->mu_c[k], rho, &w->nu[k], &w->lamc[k], &w->cdist[k], &w->cdist1[k]);
+      c->nc[k] = *nc_incld * cld_frac_l;
+
+      get_rain_dsd2(rt, *qr_incld, nr_incld, &w->mu_r[k], &w->lamr[k]);
+      get_cdistr_logn0r(*qr_incld, *nr_incld, w->mu_r[k], w->lamr[k], &w->cdistr[k], &w->logn0r[k]);
+      c->nr[k] = *nr_incld * cld_frac_r;
+
+      impose_max_total_ni(ni_incld, rt->max_total_ni, inv_rho);
+
+      if (*qi_incld >= QSMALL) {
+        *ni_incld = dmax(*ni_incld, NSMALL);
+        *nr_incld = dmax(*nr_incld, NSMALL);
+        const double rhop = calc_bulk_rho_rime(rt, *qi_incld, qm_incld, bm_incld);
+        c->qm[k] = *qm_incld * cld_frac_i;
+        c->bm[k] = *bm_incld * cld_frac_i;
+        table_ice_t ti;
+        table_rain_t tr;
+        lookup_ice(*qi_incld, *ni_incld, *qm_incld, rhop, &ti);
+        lookup_rain(*qr_incld, *nr_incld, &tr);
+        table_val_qi_fallspd = apply_table_ice(tb, 1, &ti);
+        table_val_ni_self_collect = apply_table_ice(tb, 2, &ti);
+        table_val_qc2qi_collect = apply_table_ice(tb, 3, &ti);
+        table_val_qi2qr_melting = apply_table_ice(tb, 4, &ti);
+        table_val_ni_lammax = apply_table_ice(tb, 6, &ti);
+        table_val_ni_lammin = apply_table_ice(tb, 7, &ti);
+        table_val_qi2qr_vent_melt = apply_table_ice(tb, 9, &ti);
+        if (*qr_incld >= QSMALL) {
+          table_val_nr_collect = apply_table_coll(tb, 0, &ti, &tr);
+          table_val_qr2qi_collect = apply_table_coll(tb, 1, &ti, &tr);
+        }
+        *ni_incld = dmin(*ni_incld, table_val_ni_lammax * *ni_incld);
+        *ni_incld = dmax(*ni_incld, table_val_ni_lammin * *ni_incld);
+      }
+
+      if (do_ice_production) {
+        /* ice_cldliq_collection, P3I/p3_ice_collection_impl.hpp:9-62 */
+        {
+          const int t_is_negative = T_atm <= Tmelt;
+          const int both = *qi_incld >= QSMALL && *qc_incld >= QSMALL;
+          const double f = rt->cldliq_to_ice_collection_factor;
+          if (both && t_is_negative)
+            qc2qi_collect_tend = w->rhofaci[k] * table_val_qc2qi_collect * *qc_incld * f * rho * *ni_incld;
+          if (both)
+            nc_collect_tend = w->rhofaci[k] * table_val_qc2qi_collect * *nc_incld * f * rho * *ni_incld;
+          if (both && !t_is_negative) {
+            qc2qr_ice_shed_tend = w->rhofaci[k] * table_val_qc2qi_collect * *qc_incld * f * rho * *ni_incld;
+            nc_collect_tend = w->rhofaci[k] * table_val_qc2qi_collect * *nc_incld * f * rho * *ni_incld;
+            ncshdc = qc2qr_ice_shed_tend * (1.0 / dropmass);
+          }
+        }
+        /* ice_rain_collection, P3I/p3_ice_collection_impl.hpp:64-106 */
+        {
+          const int t_is_negative = T_atm <= Tmelt;
+          const int both = *qi_incld >= QSMALL && *qr_incld >= QSMALL;
+          const double f = rt->rain_to_ice_collection_factor;
+          if (both && t_is_negative) {
+            qr2qi_collect_tend = pow(10.0, table_val_qr2qi_collect + w->logn0r[k]) * rho * w->rhofaci[k] * f * *ni_incld;
+            nr_collect_tend = pow(10.0, table_val_nr_collect + w->logn0r[k]) * rho * w->rhofaci[k] * f * *ni_incld;
+          }
+          if (both && !t_is_negative)
+            nr_collect_tend = pow(10.0, table_val_nr_collect + w->logn0r[k]) * rho * w->rhofaci[k] * f * *ni_incld;
+        }
+        /* ice_self_collection, P3I/p3_ice_collection_impl.hpp:108-151 */
+        {
+          const int qm_pos = *qm_incld > 0.0;
+          if (*qi_incld >= QSMALL) {
+            double tmp1 = 0.0, Eii_fact = 0.0;
+            if (qm_pos) tmp1 = *qm_incld / *qi_incld;
+            if (tmp1 < 0.6 && qm_pos) Eii_fact = 1;
+            if (tmp1 >= 0.6 && tmp1 < 0.9 && qm_pos) Eii_fact = 1 - (tmp1 - 0.6) / 0.3;
+            if (tmp1 >= 0.9 && qm_pos) Eii_fact = 0;
+            if (!qm_pos) Eii_fact = 1;
+            ni_selfcollect_tend = table_val_ni_self_collect * rho * eii * Eii_fact * w->rhofaci[k] * *ni_incld * *ni_incld;
+          }
+        }
+      }
+
+      /* ice_melting, P3I/p3_ice_melting_impl.hpp:11-52 */
+      if (*qi_incld >= QSMALL && T_atm > Tmelt) {
+        const double qsat0 = qv_sat_dry_e(Tmelt, pres, 0, err);
+        qi2qr_melt_tend = ((table_val_qi2qr_melting + table_val_qi2qr_vent_melt * P3_CBRT(sc) * sqrt(w->rhofaci[k] * rho / mu)) *
+                           ((T_atm - Tmelt) * kap - rho * latvap * dv * (qsat0 - c->qv[k])) * 2 * Pi / latice) * *ni_incld;
+        qi2qr_melt_tend = dmax(qi2qr_melt_tend, 0);
+        ni2nr_melt_tend = qi2qr_melt_tend * (*ni_incld / *qi_incld);
+      }
+
+      if (do_ice_production) {
+        /* ice_cldliq_wet_growth, P3I/p3_ice_cldliq_wet_growth_impl.hpp:11-79 */
+        const int t_is_negative = T_atm < Tmelt;
+        const int any_if = *qi_incld >= QSMALL && (*qc_incld + *qr_incld) >= 1.0e-6 && t_is_negative;
+        const int any_if_col = any_if && (qc2qi_collect_tend + qr2qi_collect_tend) >= 1.0e-10;
+        if (any_if) {
+          const double qsat0 = qv_sat_dry_e(Tmelt, pres, 0, err);
+          qc_growth_rate = ((table_val_qi2qr_melting + table_val_qi2qr_vent_melt * P3_CBRT(sc) * sqrt(w->rhofaci[k] * rho / mu)) *
+                            (Pi * 2) * (rho * latvap * dv * (qsat0 - c->qv[k]) - (T_atm - Tmelt) * kap) /
+                            (latice + CpLiq * (T_atm - Tmelt))) * *ni_incld;
+          qc_growth_rate = dmax(qc_growth_rate, 0.0);
+          const double dum = dmax(0.0, (qc2qi_collect_tend + qr2qi_collect_tend) - qc_growth_rate);
+          const int dum_ge_small = dum >= 1.0e-10;
+          if (dum_ge_small) {
+            nr_ice_shed_tend = nr_ice_shed_tend + dum * 1.923e+6;
+            const double dum1 = 1.0 / (qc2qi_collect_tend + qr2qi_collect_tend);
+            if (any_if_col) {
+              qc2qr_ice_shed_tend = qc2qr_ice_shed_tend + dum * qc2qi_collect_tend * dum1;
+              qc2qi_collect_tend = dmax(0, qc2qi_collect_tend - dum * qc2qi_collect_tend * dum1);
+              qr2qi_collect_tend = dmax(0, qr2qi_collect_tend - dum * qr2qi_collect_tend * dum1);
+            }
+          }
+          wetgrowth = dum_ge_small;
+        }
+      }
+
+      /* ice_relaxation_timescale, P3I/p3_ice_relaxation_timescale_impl.hpp:9-40 */
+      if (*qi_incld >= QSMALL && T_atm < Tmelt) {
+        epsi = ((table_val_qi2qr_melting + table_val_qi2qr_vent_melt * P3_CBRT(sc) * sqrt(w->rhofaci[k] * rho / mu)) *
+                2.0 * Pi * rho * dv) * *ni_incld;
+        epsi_tot = epsi_tot + epsi;
+      } else {
+        epsi = 0;
+      }
+
+      /* calc_rime_density, P3I/p3_calc_rime_density_impl.hpp:14-73 */
+      if (qc2qi_collect_tend >= QSMALL && T_atm < T_zerodegc) {
+        vtrmi1 = table_val_qi_fallspd * w->rhofaci[k];
+        if (*qc_incld >= QSMALL) {
+          const double mu_c = w->mu_c[k], lamc = w->lamc[k];
+          const double Vt_qc = w->acn[k] * tgamma(4 + bcn + mu_c) / (pow(lamc, bcn) * tgamma(4 + mu_c));
+          const double D_c = (4 + mu_c) / lamc;
+          const double V_impact = fabs(vtrmi1 - Vt_qc);
+          const double inv_Tc = 1 / dmin(-0.001, T_atm - T_zerodegc);
+          const double Ri = dmax(1, dmin(-0.5e+6 * D_c * V_impact * inv_Tc, 12));
+          if (Ri <= 8.0) rho_qm_cloud = (0.051 + 0.114 * Ri - 0.0055 * (Ri * Ri)) * 1000;
+          else rho_qm_cloud = 611. + 72.25 * (Ri - 8);
+        } else {
+          rho_qm_cloud = 400;
+        }
+      } else {
+        vtrmi1 = 0;
+        rho_qm_cloud = 400;
+      }
+
+      if (do_ice_production) {
+        /* cldliq_immersion_freezing, P3I/p3_cldliq_imm_freezing_impl.hpp:15-52 */
+        if (*qc_incld >= QSMALL && T_atm <= T_rainfrz) {
+          const double expAimmDt = exp(rt->immersion_freezing_exponent * (T_zerodegc - T_atm));
+          const double x = 1 / w->lamc[k];
+          const double inv_lamc3 = x * x * x;
+          qc2qi_hetero_freeze_tend = 1.0 * CONS6 * w->cdist1[k] * tgamma(7 + w->mu_c[k]) * expAimmDt * (inv_lamc3 * inv_lamc3);
+          nc2ni_immers_freeze_tend = CONS5 * w->cdist1[k] * tgamma(4.0 + w->mu_c[k]) * expAimmDt * inv_lamc3;
+        }
+        /* rain_immersion_freezing, P3I/p3_rain_imm_freezing_impl.hpp:14-46 */
+        if (*qr_incld >= QSMALL && T_atm <= T_rainfrz) {
+          const double mu_r = w->mu_r[k], lamr = w->lamr[k], cdistr = w->cdistr[k];
+          qr2qi_immers_freeze_tend = CONS6 * exp(log(cdistr) + log(tgamma(7. + mu_r)) - 6. * log(lamr)) *
+                                     exp(rt->immersion_freezing_exponent * (T_zerodegc - T_atm));
+          nr2ni_immers_freeze_tend = CONS5 * exp(log(cdistr) + log(tgamma(4. + mu_r)) - 3. * log(lamr)) *
+                                     exp(rt->immersion_freezing_exponent * (T_zerodegc - T_atm));
+        }
+      }
+
+      /* calc_liq_relaxation_timescale, P3I/p3_calc_liq_relaxation_timescale_impl.hpp:14-44 */
+      epsr = 0;
+      if (*qr_incld >= QSMALL) {
+        table3_t t3;
+        lookup3(w->mu_r[k], w->lamr[k], &t3);
+        epsr = 2 * Pi * w->cdistr[k] * rho * dv *
+               (f1r * tgamma(w->mu_r[k] + 2) / w->lamr[k] + f2r * sqrt(rho / mu) * P3_CBRT(sc) * apply_table3(tb->revap_table, &t3));
+      }
+      if (*qc_incld >= QSMALL) epsc = 2 * Pi * rho * dv * w->cdist[k];
+      else epsc = 0;
+
+      /* evaporate_rain, P3I/p3_evaporate_rain_impl.hpp:83-201 */
+      {
+        const double qv = c->qv[k], qv_sat_l = w->qv_sat_l[k], qv_sat_i = w->qv_sat_i[k];
+        qr2qv_evap_tend = 0;
+        nr_evap_tend = 0;
+        const double inv_dt_e = 1 / dt;
+        const double ssat_r = qv - qv_sat_l;
+        const double cld_frac = ((*qc_incld + *qi_incld) < 1.e-6) ? 0 : cld_frac_l;
+        if (*qr_incld >= QSMALL && ssat_r < 0 && cld_frac_r > cld_frac) {
+          const double tau_r = 1 / epsr;
+          double eps_eff, A_c;
+          if (T_atm < Tmelt) {
+            eps_eff = epsr + epsi_tot * (1 + (latvap + latice) * (1 / Cpair) * dqsdt) / abi;
+            A_c = (qv - c->qv_prev[k]) * inv_dt_e - dqsdt * (T_atm - c->t_prev[k]) * inv_dt_e -
+                  (qv_sat_l - qv_sat_i) * (1 + (latvap + latice) * (1 / Cpair) * dqsdt) / abi * epsi_tot;
+          } else {
+            eps_eff = epsr;
+            A_c = (qv - c->qv_prev[k]) * inv_dt_e - dqsdt * (T_atm - c->t_prev[k]) * inv_dt_e;
+          }
+          if (eps_eff < 1e-20) eps_eff = 1e-20;
+          const double tau_eff = 1 / eps_eff;
+          if (*qr_incld < 1e-12 && qv / qv_sat_l < 0.999) {
+            qr2qv_evap_tend = *qr_incld * inv_dt_e;
+          } else {
+            const double dt_over_tau = dt / tau_eff;
+            const double tscale_weight = -P3_EXPM1(-dt_over_tau) / dt_over_tau;
+            const double equilib_tend = -A_c / ab * tau_eff / tau_r;
+            const double instant_tend = -ssat_r / (ab * tau_r);
+            qr2qv_evap_tend = instant_tend * tscale_weight + equilib_tend * (1 - tscale_weight);
+          }
+          if (qr2qv_evap_tend > -ssat_r * inv_dt_e / ab) qr2qv_evap_tend = -ssat_r * inv_dt_e / ab;
+          if (qr2qv_evap_tend < 0) qr2qv_evap_tend = 0;
+          if (qr2qv_evap_tend > *qr_incld * inv_dt_e) qr2qv_evap_tend = *qr_incld * inv_dt_e;
+          qr2qv_evap_tend = qr2qv_evap_tend * (cld_frac_r - cld_frac) / cld_frac_r;
+          nr_evap_tend = qr2qv_evap_tend * (*nr_incld / *qr_incld);
+        }
+      }
+
+      if (do_ice_production) {
+        /* ice_deposition_sublimation, P3I/p3_ice_deposition_sublimation_impl.hpp:10-65 */
+        qc2qi_berg_tend = 0; qv2qi_vapdep_tend = 0; qi2qv_sublim_tend = 0; ni_sublim_tend = 0;
+        if (*qi_incld > QSMALL) {
+          const double qi_tend = dmin(epsi / abi, inv_dt) * (c->qv[k] - w->qv_sat_i[k]);
+          const int neg = qi_tend < 0;
+          if (neg) {
+            qi2qv_sublim_tend = -qi_tend;
+            ni_sublim_tend = qi2qv_sublim_tend * (*ni_incld / *qi_incld);
+          }
+          if (T_atm < T_zerodegc && !neg) qv2qi_vapdep_tend = qi_tend;
+          if (T_atm < T_zerodegc) qc2qi_berg_tend = dmax(epsi / abi * (w->qv_sat_l[k] - w->qv_sat_i[k]), 0);
+        }
+      }
+    } /* !skip_micro */
+
+    /* ice_nucleation, P3I/p3_ice_nucleation_impl.hpp:9-60 */
+    if (do_ice_production) {
+      const double mi0 = 4.0 * PIOV3 * 900.0 * 1.e-18;
+      const int lt = T_atm < (Tmelt - 15.0);
+      const int ge = w->qv_supersat_i[k] >= 0.05;
+      const int do_log = (!predictNc || prescribedCCN);
+      const int any_if_log = lt && ge && !do_log;
+      const int any_if_not_log = lt && ge && do_log;
+      if (any_if_not_log) {
+        double dum = 0.005 * exp(rt->deposition_nucleation_exponent * (Tmelt - T_atm)) * 1.0e3 * inv_rho;
+        dum = dmin(dum, 1.0e5 * inv_rho);
+        const double N_nuc = dmax(0.0, (dum - c->ni[k]) * inv_dt);
+        if (N_nuc >= NSMALL) {
+          const double Q_nuc = dmax(0.0, (dum - c->ni[k]) * mi0 * inv_dt);
+          qv2qi_nucleat_tend = Q_nuc;
+          ni_nucleat_tend = N_nuc;
+        }
+      } else if (any_if_log) {
+        ni_nucleat_tend = dmax(0.0, (c->ni_activated[k] - c->ni[k]) * inv_dt);
+        qv2qi_nucleat_tend = ni_nucleat_tend * mi0;
+      }
+    }
+
+    /* cloud_water_autoconversion, P3I/p3_autoconversion_impl.hpp:10-57 */
+    {
+      const double radius = rt->autoconversion_radius;
+      const double cons3 = (radius == 25.0e-6) ? CONS3 : 1.0 / (CONS2 * pow(radius, 3.0));
+      if (*qc_incld >= 1e-8) {
+        qc2qr_autoconv_tend = 1.0 * rt->autoconversion_prefactor * pow(*qc_incld, rt->autoconversion_qc_exponent) *
+                              pow(*nc_incld * 1.e-6 * rho, -rt->autoconversion_nc_exponent);
+        ncautr = qc2qr_autoconv_tend * cons3;
+        nc2nr_autoconv_tend = qc2qr_autoconv_tend * *nc_incld / *qc_incld;
+      }
+      if (qc2qr_autoconv_tend == 0) nc2nr_autoconv_tend = 0;
+      if (nc2nr_autoconv_tend == 0) qc2qr_autoconv_tend = 0;
+    }
+    /* droplet_self_collection, P3I/p3_droplet_self_coll_impl.hpp:14-30: rate stays 0 */
+    /* cloud_rain_accretion, P3I/p3_cloud_rain_acc_impl.hpp:15-56 */
+    if (*qr_incld >= QSMALL && *qc_incld >= QSMALL) {
+      if (rt->accretion_qc_exponent == rt->accretion_qr_exponent)
+        qc2qr_accret_tend = 1.0 * rt->accretion_prefactor * pow(*qc_incld * *qr_incld, rt->accretion_qr_exponent);
+      else
+        qc2qr_accret_tend = 1.0 * rt->accretion_prefactor * pow(*qc_incld, rt->accretion_qc_exponent) *
+                            pow(*qr_incld, rt->accretion_qr_exponent);
+      nc_accret_tend = qc2qr_accret_tend * *nc_incld / *qc_incld;
+      if (nc_accret_tend == 0) qc2qr_accret_tend = 0;
+      if (qc2qr_accret_tend == 0) nc_accret_tend = 0;
+    }
+    /* rain_self_collection, P3I/p3_rain_self_collection_impl.hpp:9-55 */
+    if (*qr_incld >= QSMALL) {
+      const double bd = rt->rain_selfcollection_breakup_diameter;
+      const double dum2 = P3_CBRT((*qr_incld) / (Pi * RHO_H2O * *nr_incld));
+      double dum;
+      if (dum2 < bd) dum = 1;
+      else dum = 2 - exp(2300 * (dum2 - bd));
+      nr_selfcollect_tend = dum * rt->rain_selfcollection_prefactor * *nr_incld * *qr_incld * rho;
+    }
+
+    /* back_to_cell_average, P3I/p3_back_to_cell_average_impl.hpp:14-76 */
+    {
+      const double ir_cldm = dmin(cld_frac_i, cld_frac_r);
+      const double il_cldm = dmin(cld_frac_i, cld_frac_l);
+      const double lr_cldm = dmin(cld_frac_l, cld_frac_r);
+      qc2qr_accret_tend *= lr_cldm;
+      qr2qv_evap_tend *= cld_frac_r;
+      qc2qr_autoconv_tend *= cld_frac_l;
+      nc_accret_tend *= lr_cldm;
+      nc_selfcollect_tend *= cld_frac_l;
+      nc2nr_autoconv_tend *= cld_frac_l;
+      nr_selfcollect_tend *= cld_frac_r;
+      nr_evap_tend *= cld_frac_r;
+      ncautr *= lr_cldm;
+      qi2qv_sublim_tend *= cld_frac_i;
+      nr_ice_shed_tend *= il_cldm;
+      qc2qi_hetero_freeze_tend *= il_cldm;
+      qr2qi_collect_tend *= ir_cldm;
+      qc2qr_ice_shed_tend *= il_cldm;
+      qi2qr_melt_tend *= cld_frac_i;
+      qc2qi_collect_tend *= il_cldm;
+      qr2qi_immers_freeze_tend *= cld_frac_r;
+      ni2nr_melt_tend *= cld_frac_i;
+      nc_collect_tend *= il_cldm;
+      ncshdc *= il_cldm;
+      nc2ni_immers_freeze_tend *= cld_frac_l;
+      nr_collect_tend *= ir_cldm;
+      ni_selfcollect_tend *= cld_frac_i;
+      qv2qi_vapdep_tend *= cld_frac_i;
+      nr2ni_immers_freeze_tend *= cld_frac_r;
+      ni_sublim_tend *= cld_frac_i;
+      qc2qi_berg_tend *= il_cldm;
+    }
+
+    /* cloud_water_conservation, P3I/p3_conservation_impl.hpp:10-40 */
+    {
+      const double sinks = (qc2qr_autoconv_tend + qc2qr_accret_tend + qc2qi_collect_tend + qc2qi_hetero_freeze_tend +
+                            qc2qr_ice_shed_tend + qc2qi_berg_tend) * dt;
+      const double sources = c->qc[k];
+      double ratio;
+      if (sinks > sources && sinks >= QTENDSMALL) {
+        ratio = sources / sinks;
+        qc2qr_autoconv_tend *= ratio;
+        qc2qr_accret_tend *= ratio;
+        qc2qi_collect_tend *= ratio;
+        qc2qi_hetero_freeze_tend *= ratio;
+        qc2qr_ice_shed_tend *= ratio;
+        qc2qi_berg_tend *= ratio;
+      } else {
+        ratio = 1;
+      }
+      if (sources > QTENDSMALL) {
+        qv2qi_vapdep_tend = qv2qi_vapdep_tend * (1 - ratio);
+        qi2qv_sublim_tend = qi2qv_sublim_tend * (1 - ratio);
+      }
+    }
+    /* rain_water_conservation, P3I/p3_conservation_impl.hpp:42-65 */
+    {
+      const double sinks = (qr2qv_evap_tend + qr2qi_collect_tend + qr2qi_immers_freeze_tend) * dt;
+      const double sources = c->qr[k] + (qc2qr_autoconv_tend + qc2qr_accret_tend + qi2qr_melt_tend + qc2qr_ice_shed_tend) * dt;
+      if (sinks > sources && sinks >= QTENDSMALL) {
+        const double ratio = sources / sinks;
+        qr2qv_evap_tend *= ratio;
+        qr2qi_collect_tend *= ratio;
+        qr2qi_immers_freeze_tend *= ratio;
+      }
+    }
+    /* ice_water_conservation, P3I/p3_conservation_impl.hpp:67-96 */
+    {
+      const double sinks = (qi2qv_sublim_tend + qi2qr_melt_tend) * dt;
+      const double sources = c->qi[k] + (qv2qi_vapdep_tend + qv2qi_nucleat_tend + qr2qi_collect_tend + qc2qi_collect_tend +
+                                         qr2qi_immers_freeze_tend + qc2qi_hetero_freeze_tend + qc2qi_berg_tend) * dt;
+      if (sinks > sources && sinks >= QTENDSMALL) {
+        const double ratio = sources / sinks;
+        qi2qv_sublim_tend *= ratio;
+        qi2qr_melt_tend *= ratio;
+      }
+    }
+    /* nc_conservation, P3I/p3_nc_conservation_impl.hpp:16-28 */
+    {
+      const double sink_nc = (nc_collect_tend + nc2ni_immers_freeze_tend + nc_accret_tend + nc2nr_autoconv_tend) * dt;
+      const double source_nc = c->nc[k] + nc_selfcollect_tend * dt;
+      if (sink_nc > source_nc) {
+        const double ratio = source_nc / sink_nc;
+        nc_collect_tend *= ratio;
+        nc2ni_immers_freeze_tend *= ratio;
+        nc_accret_tend *= ratio;
+        nc2nr_autoconv_tend *= ratio;
+      }
+    }
+    /* nr_conservation, P3I/p3_nr_conservation_impl.hpp:16-28 */
+    {
+      const double sink_nr = (nr_collect_tend + nr2ni_immers_freeze_tend + nr_selfcollect_tend + nr_evap_tend) * dt;
+      const double source_nr = c->nr[k] + (ni2nr_melt_tend * nmltratio + nr_ice_shed_tend + ncshdc + nc2nr_autoconv_tend) * dt;
+      if (sink_nr > source_nr) {
+        const double ratio = source_nr / sink_nr;
+        nr_collect_tend *= ratio;
+        nr2ni_immers_freeze_tend *= ratio;
+        nr_selfcollect_tend *= ratio;
+        nr_evap_tend *= ratio;
+      }
+    }
+    /* ni_conservation, P3I/p3_ni_conservation_impl.hpp:16-27 */
+    {
+      const double sink_ni = (ni2nr_melt_tend + ni_sublim_tend + ni_selfcollect_tend) * dt;
+      const double source_ni = c->ni[k] + (ni_nucleat_tend + nr2ni_immers_freeze_tend + nc2ni_immers_freeze_tend) * dt;
+      if (sink_ni > source_ni) {
+        const double ratio = source_ni / sink_ni;
+        ni2nr_melt_tend *= ratio;
+        ni_sublim_tend *= ratio;
+        ni_selfcollect_tend *= ratio;
+      }
+    }
+    /* ice_supersat_conservation, P3I/p3_ice_supersat_conservation_impl.hpp:14-43 */
+    {
+      const double t_atm = c->th[k] / c->inv_exner[k];
+      const double latsublim2 = (latvap + latice) * (latvap + latice);
+      const double qv_sink = qv2qi_vapdep_tend + qv2qi_nucleat_tend;
+      if (qv_sink > QSMALL && cld_frac_i > 1e-20) {
+        double qv_avail = (c->qv[k] + (qi2qv_sublim_tend + qr2qv_evap_tend) * dt - w->qv_sat_i[k]) /
+                          (1 + latsublim2 * w->qv_sat_i[k] / (Cpair * RH2O * (t_atm * t_atm))) / dt;
+        qv_avail = dmax(qv_avail, 0);
+        if (qv_sink > qv_avail) {
+          const double fract = qv_avail / qv_sink;
+          qv2qi_nucleat_tend = qv2qi_nucleat_tend * fract;
+          qv2qi_vapdep_tend = qv2qi_vapdep_tend * fract;
+        }
+      }
+    }
+    /* prevent_liq_supersaturation, P3I/p3_prevent_liq_supersaturation_impl.hpp:15-78 */
+    {
+      const double qv_sources = qi2qv_sublim_tend + qr2qv_evap_tend;
+      if (qv_sources >= QSMALL) {
+        const double qv_sinks = qv2qi_vapdep_tend + qv2qi_nucleat_tend;
+        const double T_endstep = T_atm + ((qv_sinks - qi2qv_sublim_tend) * (latvap + latice) * inv_cp -
+                                          qr2qv_evap_tend * latvap * inv_cp) * dt;
+        const double qsl = qv_sat_dry_e(T_endstep, pres, 0, err);
+        const double A = latvap * qsl * dt * inv_cp / (RV * T_endstep * T_endstep) *
+                         ((latvap + latice) * qi2qv_sublim_tend + latvap * qr2qv_evap_tend);
+        double frac = (qsl - c->qv[k] + qv_sinks * dt + A) / (qv_sources * dt + A);
+        frac = dmax(0, frac);
+        frac = dmin(1, frac);
+        qi2qv_sublim_tend = frac * qi2qv_sublim_tend;
+        qr2qv_evap_tend = frac * qr2qv_evap_tend;
+      }
+    }
+
+    /* update_prognostic_ice, P3I/p3_update_prognostics_impl.hpp:9-87 */
+    {
+      double *qc = &c->qc[k], *nc = &c->nc[k], *qr = &c->qr[k], *nr = &c->nr[k];
+      double *qi = &c->qi[k], *ni = &c->ni[k], *qm = &c->qm[k], *bm = &c->bm[k];
+      double *qv = &c->qv[k], *th = &c->th[k];
+      const double inv_exner = c->inv_exner[k];
+      *qc = *qc + (-qc2qi_hetero_freeze_tend - qc2qi_collect_tend - qc2qr_ice_shed_tend - qc2qi_berg_tend) * dt;
+      if (predictNc) *nc = *nc + (-nc_collect_tend - nc2ni_immers_freeze_tend) * dt;
+      *qr = *qr + (-qr2qi_collect_tend + qi2qr_melt_tend - qr2qi_immers_freeze_tend + qc2qr_ice_shed_tend) * dt;
+      *nr = *nr + (-nr_collect_tend - nr2ni_immers_freeze_tend + nmltratio * ni2nr_melt_tend + nr_ice_shed_tend + ncshdc) * dt;
+      if (*qi >= QSMALL) {
+        *bm = *bm - ((qi2qv_sublim_tend + qi2qr_melt_tend) / *qi) * dt * *bm;
+        *qm = *qm - ((qi2qv_sublim_tend + qi2qr_melt_tend) * *qm / *qi) * dt;
+        *qi = *qi - (qi2qv_sublim_tend + qi2qr_melt_tend) * dt;
+      }
+      const double dum = (qr2qi_collect_tend + qc2qi_collect_tend + qr2qi_immers_freeze_tend + qc2qi_hetero_freeze_tend) * dt;
+      *qi = *qi + (qv2qi_vapdep_tend + qv2qi_nucleat_tend + qc2qi_berg_tend) * dt + dum;
+      *qm = *qm + dum;
+      *bm = *bm + (qr2qi_collect_tend * INV_RHO_RIMEMAX + qc2qi_collect_tend / rho_qm_cloud +
+                   (qr2qi_immers_freeze_tend + qc2qi_hetero_freeze_tend) * INV_RHO_RIMEMAX) * dt;
+      *ni = *ni + (ni_nucleat_tend - ni2nr_melt_tend - ni_sublim_tend - ni_selfcollect_tend + nr2ni_immers_freeze_tend +
+                   nc2ni_immers_freeze_tend) * dt;
+      if (*qm < 0) { *qm = 0; *bm = 0; }
+      if (wetgrowth) { *qm = *qi; *bm = *qm * INV_RHO_RIMEMAX; }
+      *qv = *qv + (-qv2qi_vapdep_tend + qi2qv_sublim_tend - qv2qi_nucleat_tend) * dt;
+      *th = *th + inv_exner * ((qv2qi_vapdep_tend - qi2qv_sublim_tend + qv2qi_nucleat_tend) * (latvap + latice) * INV_CP +
+                               (qr2qi_collect_tend + qc2qi_collect_tend + qc2qi_hetero_freeze_tend + qr2qi_immers_freeze_tend -
+                                qi2qr_melt_tend + qc2qi_berg_tend) * latice * INV_CP) * dt;
+
+      /* update_prognostic_liquid, P3I/p3_update_prognostics_impl.hpp:89-122 */
+      *qc = *qc + (-qc2qr_accret_tend - qc2qr_autoconv_tend) * dt;
+      *qr = *qr + (qc2qr_accret_tend + qc2qr_autoconv_tend - qr2qv_evap_tend) * dt;
+      if (predictNc || prescribedCCN) *nc = *nc + (-nc_accret_tend - nc2nr_autoconv_tend + nc_selfcollect_tend) * dt;
+      else *nc = NCCNST * inv_rho;
+      *nr = *nr + (ncautr - nr_selfcollect_tend - nr_evap_tend) * dt;
+      *qv = *qv + qr2qv_evap_tend * dt;
+      *th = *th + inv_exner * (-qr2qv_evap_tend * latvap * INV_CP) * dt;
+
+      /* extra outputs, P3I/p3_main_impl_part2.hpp:456-463 */
+      c->qv2qi_depos_tend[k] = qv2qi_vapdep_tend - qi2qv_sublim_tend + qv2qi_nucleat_tend;
+      c->precip_total_tend[k] = qc2qr_accret_tend + qc2qr_autoconv_tend + qc2qr_ice_shed_tend + qc2qi_collect_tend;
+      c->nevapr[k] = qi2qv_sublim_tend + qr2qv_evap_tend;
+      c->vap_ice_exchange[k] = qv2qi_vapdep_tend - qi2qv_sublim_tend + qv2qi_nucleat_tend;
+      c->vap_liq_exchange[k] = -qr2qv_evap_tend;
+      c->liq_ice_exchange[k] = qc2qi_hetero_freeze_tend + qr2qi_immers_freeze_tend - qi2qr_melt_tend + qc2qi_berg_tend +
+                               qc2qi_collect_tend + qr2qi_collect_tend;
+
+      /* clipping, P3I/p3_main_impl_part2.hpp:465-508 */
+      if (*qc < QSMALL) {
+        *qv = *qv + *qc;
+        *th = *th - inv_exner * *qc * latvap * inv_cp;
+        *qc = 0; *nc = 0;
+      } else hydrometeorsPresent = 1;
+      if (*qr < QSMALL) {
+        *qv = *qv + *qr;
+        *th = *th - inv_exner * *qr * latvap * inv_cp;
+        *qr = 0; *nr = 0;
+      } else hydrometeorsPresent = 1;
+      if (*qi < QSMALL) {
+        *qv = *qv + *qi;
+        *th = *th - inv_exner * *qi * (latvap + latice) * inv_cp;
+        *qi = 0; *ni = 0; *qm = 0; *bm = 0;
+      } else hydrometeorsPresent = 1;
+
+      /* P3I/p3_main_impl_part2.hpp:514-522 */
+      *ni_incld = *ni / cld_frac_i;
+      impose_max_total_ni(ni_incld, rt->max_total_ni, inv_rho);
+      *ni = *ni_incld * cld_frac_i;
+      incloud_mixingratios(*qc, *qr, *qi, *qm, *nc, *nr, *ni, *bm, w->icfl[k], w->icfi[k], w->icfr[k],
+                           qc_incld, qr_incld, qi_incld, qm_incld, nc_incld, nr_incld, ni_incld, bm_incld);
+    }
+  }
+
+  c->status = 1;
+  if (!hydrometeorsPresent) return;
+  c->status = 2;
+
+  /* ---- cloud_sedimentation, P3I/p3_cloud_sed_impl.hpp:14-147 ---- */
+  {
+    int present;
+    const int k_qxtop = find_top(c->qc, QSMALL, nk, &present);
+    if (present) {
+      double dt_left = dt, prt_accum = 0.0;
+      int k_qxbot = find_bottom(c->qc, QSMALL, kbot, k_qxtop, &present);
+      while (dt_left > dt_left_tol) {
+        double Co_max = 0.0;
+        for (int k = 0; k < nk; ++k) { w->V1[k] = 0; if (predictNc) w->V2[k] = 0; }
+        for (int k = k_qxtop; k <= k_qxbot; ++k) {
+          double Co = 0;
+          if (w->qc_incld[k] > QSMALL) {
+            double nu, cdist, cdist1;
+            get_cloud_dsd2(w->qc_incld[k], &w->nc_incld[k], &w->mu_c[k], w->rho[k], &nu, &w->lamc[k], &cdist, &cdist1);
+            c->nc[k] = w->nc_incld[k] * c->cld_frac_l[k];
+            const double dum = 1 / pow(w->lamc[k], bcn);
+            w->V1[k] = w->acn[k] * tgamma(4 + bcn + w->mu_c[k]) * dum / tgamma(w->mu_c[k] + 4);
+            if (predictNc) w->V2[k] = w->acn[k] * tgamma(1 + bcn + w->mu_c[k]) * dum / tgamma(w->mu_c[k] + 1);
+            Co = dmax(0, w->V1[k] * dt_left * w->inv_dz[k]);
+          }
+          if (Co > Co_max) Co_max = Co;
+        }
+        double* fl[2] = {w->F1, w->F2};
+        const double* vv[2] = {w->V1, w->V2};
+        double* rr[2] = {c->qc, c->nc};
+        generalized_sedimentation(nk, predictNc ? 2 : 1, w->rho, w->inv_rho, w->inv_dz, k_qxtop, &k_qxbot, kbot,
+                                  Co_max, &dt_left, &prt_accum, fl, vv, rr);
+        c->nsub[0]++;
+        for (int k = 0; k < nk; ++k) {
+          w->qc_incld[k] = c->qc[k] / c->cld_frac_l[k];
+          w->nc_incld[k] = c->nc[k] / c->cld_frac_l[k];
+        }
+      }
+      c->precip_liq_surf = prt_accum * INV_RHO_H2O * inv_dt;
+    }
+  }
+
+  /* ---- rain_sedimentation, P3I/p3_rain_sed_impl.hpp:36-170 ---- */
+  {
+    int present;
+    const int k_qxtop = find_top(c->qr, QSMALL, nk, &present);
+    if (present) {
+      double dt_left = dt, prt_accum = 0.0;
+      int k_qxbot = find_bottom(c->qr, QSMALL, kbot, k_qxtop, &present);
+      while (dt_left > dt_left_tol) {
+        double Co_max = 0.0;
+        for (int k = 0; k < nk; ++k) { w->V1[k] = 0; w->V2[k] = 0; }
+        for (int k = k_qxtop; k <= k_qxbot; ++k) {
+          double Co = 0;
+          if (w->qr_incld[k] > QSMALL) {
+            /* compute_rain_fall_velocity, P3I/p3_rain_sed_impl.hpp:14-34 */
+            get_rain_dsd2(rt, w->qr_incld[k], &w->nr_incld[k], &w->mu_r[k], &w->lamr[k]);
+            table3_t t3;
+            lookup3(w->mu_r[k], w->lamr[k], &t3);
+            w->V1[k] = apply_table3(tb->vm_table, &t3) * w->rhofacr[k];
+            w->V2[k] = apply_table3(tb->vn_table, &t3) * w->rhofacr[k];
+            c->nr[k] = w->nr_incld[k] * c->cld_frac_r[k];
+            Co = dmax(0, w->V1[k] * dt_left * w->inv_dz[k]);
+          }
+          if (Co > Co_max) Co_max = Co;
+        }
+        double* fl[2] = {w->F1, w->F2};
+        const double* vv[2] = {w->V1, w->V2};
+        double* rr[2] = {c->qr, c->nr};
+        generalized_sedimentation(nk, 2, w->rho, w->inv_rho, w->inv_dz, k_qxtop, &k_qxbot, kbot, Co_max,
+                                  &dt_left, &prt_accum, fl, vv, rr);
+        c->nsub[1]++;
+        for (int k = 0; k < nk; ++k) {
+          w->qr_incld[k] = c->qr[k] / c->cld_frac_r[k];
+          w->nr_incld[k] = c->nr[k] / c->cld_frac_r[k];
+        }
+        for (int k = k_qxtop + 1; k <= k_qxbot + 1; ++k)
+          c->precip_liq_flux[k] = c->precip_liq_flux[k] + w->F1[k - 1];
+      }
+      c->precip_liq_surf += prt_accum * INV_RHO_H2O * inv_dt;
+    }
+  }
+
+  /* ---- ice_sedimentation, P3I/p3_ice_sed_impl.hpp:66-205 ---- */
+  {
+    int present;
+    const int k_qxtop = find_top(c->qi, QSMALL, nk, &present);
+    if (present) {
+      double dt_left = dt, prt_accum = 0.0;
+      int k_qxbot = find_bottom(c->qi, QSMALL, kbot, k_qxtop, &present);
+      while (dt_left > dt_left_tol) {
+        double Co_max = 0.0;
+        for (int k = 0; k < nk; ++k) { w->V1[k] = 0; w->V2[k] = 0; }
+        for (int k = k_qxtop; k <= k_qxbot; ++k) {
+          double Co = 0;
+          if (w->qi_incld[k] > QSMALL) {
+            w->ni_incld[k] = dmax(w->ni_incld[k], NSMALL);
+            const double rhop = calc_bulk_rho_rime(rt, w->qi_incld[k], &w->qm_incld[k], &w->bm_incld[k]);
+            c->qm[k] = w->qm_incld[k] * c->cld_frac_i[k];
+            c->bm[k] = w->bm_incld[k] * c->cld_frac_i[k];
+            table_ice_t ti;
+            lookup_ice(w->qi_incld[k], w->ni_incld[k], w->qm_incld[k], rhop, &ti);
+            const double t_ni_fall = apply_table_ice(tb, 0, &ti);
+            const double t_qi_fall = apply_table_ice(tb, 1, &ti);
+            const double t_lammax = apply_table_ice(tb, 6, &ti);
+            const double t_lammin = apply_table_ice(tb, 7, &ti);
+            w->ni_incld[k] = dmin(w->ni_incld[k], t_lammax * w->ni_incld[k]);
+            w->ni_incld[k] = dmax(w->ni_incld[k], t_lammin * w->ni_incld[k]);
+            c->ni[k] = w->ni_incld[k] * c->cld_frac_i[k];
+            w->V1[k] = rt->ice_sedimentation_factor * t_qi_fall * w->rhofaci[k];
+            w->V2[k] = rt->ice_sedimentation_factor * t_ni_fall * w->rhofaci[k];
+            Co = dmax(0, w->V1[k] * dt_left * w->inv_dz[k]);
+          }
+          if (Co > Co_max) Co_max = Co;
+        }
+        double* fl[4] = {w->F1, w->F2, w->F3, w->F4};
+        const double* vv[4] = {w->V1, w->V2, w->V1, w->V1};
+        double* rr[4] = {c->qi, c->ni, c->qm, c->bm};
+        generalized_sedimentation(nk, 4, w->rho, w->inv_rho, w->inv_dz, k_qxtop, &k_qxbot, kbot, Co_max,
+                                  &dt_left, &prt_accum, fl, vv, rr);
+        c->nsub[2]++;
+        for (int k = 0; k < nk; ++k) {
+          w->qi_incld[k] = c->qi[k] / c->cld_frac_i[k];
+          w->ni_incld[k] = c->ni[k] / c->cld_frac_i[k];
+          w->qm_incld[k] = c->qm[k] / c->cld_frac_i[k];
+          w->bm_incld[k] = c->bm[k] / c->cld_frac_i[k];
+        }
+      }
+      c->precip_ice_surf += prt_accum * INV_RHO_H2O * inv_dt;
+    }
+  }
+
+  /* ---- homogeneous_freezing, P3I/p3_ice_sed_impl.hpp:207-270 ---- */
+  if (do_ice_production) {
+    for (int k = 0; k < nk; ++k) {
+      const int t_lt = w->T_atm[k] < T_homogfrz;
+      const int qc_ge = t_lt && c->qc[k] >= QSMALL;
+      const int qr_ge = t_lt && c->qr[k] >= QSMALL;
+      const double Qc_nuc = c->qc[k], Qr_nuc = c->qr[k];
+      const double Nc_nuc = dmax(c->nc[k], NSMALL), Nr_nuc = dmax(c->nr[k], NSMALL);
+      if (qc_ge) {
+        c->qm[k] = c->qm[k] + Qc_nuc;
+        c->qi[k] = c->qi[k] + Qc_nuc;
+        c->bm[k] = c->bm[k] + Qc_nuc * INV_RHO_RIMEMAX;
+        c->ni[k] = c->ni[k] + Nc_nuc;
+        c->th[k] = c->th[k] + c->inv_exner[k] * Qc_nuc * latice * inv_cp;
+      }
+      if (qr_ge) {
+        c->qm[k] = c->qm[k] + Qr_nuc;
+        c->qi[k] = c->qi[k] + Qr_nuc;
+        c->bm[k] = c->bm[k] + Qr_nuc * INV_RHO_RIMEMAX;
+        c->ni[k] = c->ni[k] + Nr_nuc;
+        c->th[k] = c->th[k] + c->inv_exner[k] * Qr_nuc * latice * inv_cp;
+      }
+      if (qc_ge) { c->qc[k] = 0; c->nc[k] = 0; }
+      if (qr_ge) { c->qr[k] = 0; c->nr[k] = 0; }
+    }
+  }
+
+  /* ---- p3_main_part3, P3I/p3_main_impl_part3.hpp:18-204 ---- */
+  for (int k = 0; k < nk; ++k) {
+    const double rho = w->rho[k], inv_exner = c->inv_exner[k];
+    const double cfl = c->cld_frac_l[k], cfr = c->cld_frac_r[k], cfi = c->cld_frac_i[k];
+    /* cloud */
+    {
+      const int qc_gt = c->qc[k] >= QSMALL;
+      const double qc_incld = c->qc[k] / cfl;
+      double nc_incld = c->nc[k] / cfl;
+      if (qc_gt) {
+        double ign1, ign2;
+        get_cloud_dsd2(qc_incld, &nc_incld, &w->mu_c[k], rho, &w->nu[k], &w->lamc[k], &ign1, &ign2);
+        c->nc[k] = nc_incld * cfl;
+        c->diag_eff_radius_qc[k] = 0.5 * (w->mu_c[k] + 3) / w->lamc[k];
+      } else {
+        c->qv[k] = c->qv[k] + c->qc[k];
+        c->th[k] = c->th[k] - inv_exner * c->qc[k] * latvap * inv_cp;
+        c->vap_liq_exchange[k] = c->vap_liq_exchange[k] - c->qc[k];
+        c->qc[k] = 0;
+        c->nc[k] = 0;
+      }
+    }
+    /* rain */
+    {
+      const int qr_gt = c->qr[k] >= QSMALL;
+      const double qr_incld = c->qr[k] / cfr;
+      double nr_incld = c->nr[k] / cfr;
+      if (qr_gt) {
+        get_rain_dsd2(rt, qr_incld, &nr_incld, &w->mu_r[k], &w->lamr[k]);
+        c->nr[k] = nr_incld * cfr;
+        const double m = w->mu_r[k];
+        w->ze_rain[k] = c->nr[k] * (m + 6) * (m + 5) * (m + 4) * (m + 3) * (m + 2) * (m + 1) / pow(w->lamr[k], 6.0);
+        w->ze_rain[k] = dmax(w->ze_rain[k], 1.e-22);
+        c->diag_eff_radius_qr[k] = 1.5 / w->lamr[k];
+      } else {
+        c->qv[k] = c->qv[k] + c->qr[k];
+        c->th[k] = c->th[k] - inv_exner * c->qr[k] * latvap * inv_cp;
+        c->vap_liq_exchange[k] = c->vap_liq_exchange[k] - c->qr[k];
+        c->qr[k] = 0;
+        c->nr[k] = 0;
+      }
+    }
+    /* ice */
+    {
+      const int qi_gt = c->qi[k] >= QSMALL;
+      c->ni[k] = dmax(c->ni[k], NSMALL);
+      const double qi_incld = c->qi[k] / cfi;
+      double ni_incld = c->ni[k] / cfi;
+      double qm_incld = c->qm[k] / cfi;
+      double bm_incld = c->bm[k] / cfi;
+      double rhop = 0;
+      if (qi_gt) {
+        rhop = calc_bulk_rho_rime(rt, qi_incld, &qm_incld, &bm_incld);
+        c->qm[k] = qm_incld * cfi;
+        c->bm[k] = bm_incld * cfi;
+      }
+      impose_max_total_ni(&ni_incld, rt->max_total_ni, w->inv_rho[k]);
+      double tv_qi_fallspd = 0, tv_eff = 0, tv_lammax = 0, tv_lammin = 0, tv_refl = 0, tv_dmm = 0, tv_rhomm = 0;
+      if (qi_gt) {
+        table_ice_t ti;
+        lookup_ice(qi_incld, ni_incld, qm_incld, rhop, &ti);
+        if (c->ice_idx) c->ice_idx[k] = ti.dumjj * 10000 + ti.dumii * 100 + ti.dumi;
+        tv_qi_fallspd = apply_table_ice(tb, 1, &ti);
+        tv_eff = apply_table_ice(tb, 5, &ti);
+        tv_lammax = apply_table_ice(tb, 6, &ti);
+        tv_lammin = apply_table_ice(tb, 7, &ti);
+        tv_refl = apply_table_ice(tb, 8, &ti);
+        tv_dmm = apply_table_ice(tb, 10, &ti);
+        tv_rhomm = apply_table_ice(tb, 11, &ti);
+        ni_incld = dmin(ni_incld, tv_lammax * ni_incld);
+        ni_incld = dmax(ni_incld, tv_lammin * ni_incld);
+      }
+      c->ni[k] = ni_incld * cfi;
+      if (c->qm[k] < QSMALL && qi_gt) { c->qm[k] = 0; c->bm[k] = 0; }
+      if (qi_gt) {
+        w->diag_vm_qi[k] = tv_qi_fallspd * w->rhofaci[k];
+        c->diag_eff_radius_qi[k] = tv_eff;
+        w->diag_diam_qi[k] = tv_dmm;
+        c->rho_qi[k] = tv_rhomm;
+        w->ze_ice[k] = w->ze_ice[k] + 0.1892 * tv_refl * ni_incld * rho;
+        w->ze_ice[k] = dmax(w->ze_ice[k], 1.e-22);
+        w->ze_ice[k] = w->ze_ice[k] * cfi;
+      } else {
+        c->qv[k] = c->qv[k] + c->qi[k];
+        c->th[k] = c->th[k] - inv_exner * c->qi[k] * (latvap + latice) * inv_cp;
+        c->qi[k] = 0; c->ni[k] = 0; c->qm[k] = 0; c->bm[k] = 0;
+        w->diag_diam_qi[k] = 0;
+      }
+    }
+    w->diag_equiv_reflectivity[k] = 10 * log10((w->ze_rain[k] + w->ze_ice[k]) * 1.e18);
+    if (c->qr[k] < QSMALL) c->nr[k] = 0;
+  }
+}
+
+/* ============================ driver ============================ */
+
+void p3o_runtime_defaults(p3o_runtime* rt) {
+  rt->max_total_ni = 740.0e3;
+  rt->autoconversion_prefactor = 1350.0;
+  rt->autoconversion_qc_exponent = 2.47;
+  rt->autoconversion_nc_exponent = 1.79;
+  rt->autoconversion_radius = 25.0e-6;
+  rt->accretion_prefactor = 67.0;
+  rt->accretion_qc_exponent = 1.15;
+  rt->accretion_qr_exponent = 1.15;
+  rt->rain_selfcollection_prefactor = 5.78;
+  rt->rain_selfcollection_breakup_diameter = 0.00028;
+  rt->constant_mu_rain = 1.0;
+  rt->spa_ccn_to_nc_factor = 1.0;
+  rt->cldliq_to_ice_collection_factor = 0.5;
+  rt->rain_to_ice_collection_factor = 1.0;
+  rt->min_rime_rho = 50.0;
+  rt->max_rime_rho = 900.0;
+  rt->immersion_freezing_exponent = 0.65;
+  rt->deposition_nucleation_exponent = 0.304;
+  rt->ice_sedimentation_factor = 1.0;
+  rt->do_ice_production = 1;
+}
+
+#define NCOLF 36
+
+int p3o_main(const p3o_runtime* rt, const p3o_tables* tab, const p3o_fields* f,
+             const p3o_debug* dbg, int ncol, int nlev, double dt,
+             int predictNc, int prescribedCCN, int nthreads) {
+  int nerr = 0;
+  const int nk = nlev;
+#ifdef _OPENMP
+  if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel num_threads(nthreads) reduction(+ : nerr)
+#endif
+  {
+    col_ws w;
+    ws_alloc(&w, nk);
+    col_io c;
+    c.mem = (double*)calloc((size_t)NCOLF * (nk + 1), sizeof(double));
+    double** p[] = {&c.qc, &c.nc, &c.qr, &c.nr, &c.qi, &c.qm, &c.ni, &c.bm, &c.qv, &c.th,
+                    &c.nc_nuceat_tend, &c.nccn, &c.ni_activated, &c.inv_qc_relvar, &c.cld_frac_i,
+                    &c.cld_frac_l, &c.cld_frac_r, &c.pres, &c.dz, &c.dpres, &c.inv_exner,
+                    &c.qv_prev, &c.t_prev, &c.qv2qi_depos_tend, &c.diag_eff_radius_qc,
+                    &c.diag_eff_radius_qi, &c.diag_eff_radius_qr, &c.rho_qi, &c.precip_liq_flux,
+                    &c.precip_ice_flux, &c.precip_total_tend, &c.nevapr, &c.liq_ice_exchange,
+                    &c.vap_liq_exchange, &c.vap_ice_exchange};
+    const int np = (int)(sizeof(p) / sizeof(p[0]));
+    for (int j = 0; j < np; ++j) *p[j] = c.mem + (size_t)j * (nk + 1);
+    int* ice_idx = (int*)malloc(sizeof(int) * (size_t)nk);
+
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 16)
+#endif
+    for (int i = 0; i < ncol; ++i) {
+      /* gather column i (level-major, stride ncol) */
+      const double* const in_src[] = {f->qc, f->nc, f->qr, f->nr, f->qi, f->qm, f->ni, f->bm, f->qv, f->th,
+                                      f->nc_nuceat_tend, f->nccn, f->ni_activated, f->inv_qc_relvar,
+                                      f->cld_frac_i, f->cld_frac_l, f->cld_frac_r, f->pres, f->dz,
+                                      f->dpres, f->inv_exner, f->qv_prev, f->t_prev};
+      double* const in_dst[] = {c.qc, c.nc, c.qr, c.nr, c.qi, c.qm, c.ni, c.bm, c.qv, c.th,
+                                c.nc_nuceat_tend, c.nccn, c.ni_activated, c.inv_qc_relvar,
+                                c.cld_frac_i, c.cld_frac_l, c.cld_frac_r, c.pres, c.dz,
+                                c.dpres, c.inv_exner, c.qv_prev, c.t_prev};
+      for (int j = 0; j < 23; ++j)
+        for (int k = 0; k < nk; ++k) in_dst[j][k] = in_src[j][(size_t)k * ncol + i];
+      for (int k = 0; k < nk; ++k) {
+        c.liq_ice_exchange[k] = f->liq_ice_exchange[(size_t)k * ncol + i];
+        c.vap_liq_exchange[k] = f->vap_liq_exchange[(size_t)k * ncol + i];
+        c.vap_ice_exchange[k] = f->vap_ice_exchange[(size_t)k * ncol + i];
+        ice_idx[k] = -1;
+      }
+      c.ice_idx = ice_idx;
+      int err = 0;
+      p3_column(rt, tab, nk, dt, predictNc, prescribedCCN, &c, &w, &err);
+      nerr += err;
+      /* scatter */
+      double* const out_dst[] = {f->qc, f->nc, f->qr, f->nr, f->qi, f->qm, f->ni, f->bm, f->qv, f->th,
+                                 f->qv2qi_depos_tend, f->diag_eff_radius_qc, f->diag_eff_radius_qi,
+                                 f->diag_eff_radius_qr, f->rho_qi, f->precip_total_tend, f->nevapr,
+                                 f->liq_ice_exchange, f->vap_liq_exchange, f->vap_ice_exchange};
+      const double* const out_src[] = {c.qc, c.nc, c.qr, c.nr, c.qi, c.qm, c.ni, c.bm, c.qv, c.th,
+                                       c.qv2qi_depos_tend, c.diag_eff_radius_qc, c.diag_eff_radius_qi,
+                                       c.diag_eff_radius_qr, c.rho_qi, c.precip_total_tend, c.nevapr,
+                                       c.liq_ice_exchange, c.vap_liq_exchange, c.vap_ice_exchange};
+      for (int j = 0; j < 20; ++j)
+        for (int k = 0; k < nk; ++k) out_dst[j][(size_t)k * ncol + i] = out_src[j][k];
+      for (int k = 0; k <= nk; ++k) {
+        f->precip_liq_flux[(size_t)k * ncol + i] = c.precip_liq_flux[k];
+        f->precip_ice_flux[(size_t)k * ncol + i] = c.precip_ice_flux[k];
+      }
+      f->precip_liq_surf[i] = c.precip_liq_surf;
+      f->precip_ice_surf[i] = c.precip_ice_surf;
+      if (dbg) {
+        if (dbg->col_status) dbg->col_status[i] = c.status;
+        if (dbg->sed_substeps)
+          for (int s = 0; s < 3; ++s) dbg->sed_substeps[s * ncol + i] = c.nsub[s];
+        if (dbg->ice_table_idx)
+          for (int k = 0; k < nk; ++k) dbg->ice_table_idx[(size_t)k * ncol + i] = ice_idx[k];
+      }
+    }
+    free(ice_idx);
+    free(c.mem);
+    free(w.mem);
+  }
+  return nerr;
+}
