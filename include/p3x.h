@@ -1,0 +1,189 @@
+/*
+ * p3x.h -- C ABI of the MI355X P3 microphysics step (drop-in for EAMxx p3_main).
+ *
+ * Reference interface replaced (paths under components/eamxx/src/physics/p3/):
+ *   Functions<Real,Device>::p3_main(runtime, prog, diag_in, diag_out, infra,
+ *       hist, tables, workspace_mgr, nj, nk)            p3_functions.hpp:1356-1369
+ *   P3Runtime                                           p3_functions.hpp:112-158
+ *   P3PrognosticState                                   p3_functions.hpp:161-183
+ *   P3DiagnosticInputs                                  p3_functions.hpp:186-214
+ *   P3DiagnosticOutputs                                 p3_functions.hpp:217-241
+ *   P3Infrastructure                                    p3_functions.hpp:244-264
+ *   P3HistoryOnly                                       p3_functions.hpp:268-276
+ *   P3LookupTables                                      p3_functions.hpp:279-290
+ *   init_kokkos_ice_lookup_tables / init_kokkos_tables  impl/p3_table_ice_impl.hpp:16-78,
+ *                                                       impl/p3_table3_impl.hpp:85-148
+ *
+ * Data layout.  Every per-level field is a DEVICE pointer to a level-major
+ * array: element (column i, level k) lives at [k*ld + i], k = 0 at the model
+ * top, 0 <= i < nj, ld >= nj.  The two precipitation-flux fields have nk+1
+ * rows (interfaces).  precip_liq_surf / precip_ice_surf hold nj values.
+ * (The reference's Kokkos views are (column, pack) LayoutRight; the level-major
+ * layout is what lets consecutive GPU lanes read consecutive columns.)
+ *
+ * Field names, units and meanings are those of the reference structs.
+ * Semantics follow the reference's p3_main exactly, with one documented
+ * choice: all nk+1 entries of precip_liq_flux / precip_ice_flux are zeroed
+ * (as the Fortran reference does); see p3x_infrastructure for the exchange
+ * fields.
+ *
+ * Thread safety: one context may be used by one host thread at a time.
+ * Every call is enqueued on the stream given in p3x_launch (default stream if
+ * NULL) and, unless p3x_launch.async is set, is complete on return.
+ */
+#ifndef P3X_H
+#define P3X_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+enum {
+  P3X_OK = 0,
+  P3X_ERR_ARG = 1,          /* null pointer, bad size or bad option                 */
+  P3X_ERR_TABLE_FILE = 2,   /* cannot open / parse the ice lookup table             */
+  P3X_ERR_TABLE_VERSION = 3,/* table header is not "VERSION 4.1.1"                 */
+  P3X_ERR_DEVICE = 4,       /* a HIP runtime call failed                           */
+  P3X_ERR_TEMPERATURE = 5,  /* T <= 0 or NaN reached the saturation functions;     */
+                            /* the reference aborts here (physics_saturation_impl.hpp:14-36) */
+  P3X_ERR_NO_MEMORY = 6,    /* device allocation failed                            */
+  P3X_ERR_SUBSTEPS = 7      /* a sedimentation loop hit the 100000-substep cap     */
+};
+
+/* ---- P3Runtime (p3_functions.hpp:112-158); defaults from p3x_runtime_defaults ---- */
+typedef struct p3x_runtime {
+  double max_total_ni;
+  double autoconversion_prefactor;
+  double autoconversion_qc_exponent;
+  double autoconversion_nc_exponent;
+  double autoconversion_radius;
+  double accretion_prefactor;
+  double accretion_qc_exponent;
+  double accretion_qr_exponent;
+  double rain_selfcollection_prefactor;
+  double rain_selfcollection_breakup_diameter;
+  double constant_mu_rain;
+  double spa_ccn_to_nc_factor;
+  double cldliq_to_ice_collection_factor;
+  double rain_to_ice_collection_factor;
+  double min_rime_rho;
+  double max_rime_rho;
+  double immersion_freezing_exponent;
+  double deposition_nucleation_exponent;
+  double ice_sedimentation_factor;
+  int do_ice_production;
+} p3x_runtime;
+
+/* ---- P3PrognosticState (p3_functions.hpp:161-183), in/out ---- */
+typedef struct p3x_prognostic_state {
+  double *qc, *nc, *qr, *nr, *qi, *qm, *ni, *bm, *qv, *th;
+} p3x_prognostic_state;
+
+/* ---- P3DiagnosticInputs (p3_functions.hpp:186-214) ---- */
+typedef struct p3x_diagnostic_inputs {
+  const double *nc_nuceat_tend, *nccn, *ni_activated, *inv_qc_relvar;
+  const double *cld_frac_i, *cld_frac_l, *cld_frac_r;
+  const double *pres, *dz, *dpres, *inv_exner, *qv_prev, *t_prev;
+} p3x_diagnostic_inputs;
+
+/* ---- P3DiagnosticOutputs (p3_functions.hpp:217-241) ---- */
+typedef struct p3x_diagnostic_outputs {
+  double *qv2qi_depos_tend;
+  double *precip_liq_surf, *precip_ice_surf;             /* nj values          */
+  double *diag_eff_radius_qc, *diag_eff_radius_qi, *diag_eff_radius_qr;
+  double *rho_qi;
+  double *precip_liq_flux, *precip_ice_flux;             /* nk+1 rows          */
+  double *precip_total_tend, *nevapr;
+} p3x_diagnostic_outputs;
+
+/* ---- P3Infrastructure (p3_functions.hpp:244-264) ----
+ * its/ite/kts/kte and col_location are carried for interface parity; the step
+ * uses nj/nk of p3x_main.  The reference's p3_main leaves the three exchange
+ * fields of P3HistoryOnly untouched where a column or level is skipped and
+ * accumulates into vap_liq_exchange in part 3, so its caller
+ * (P3Microphysics::run_impl, eamxx_p3_run.cpp:27-29) zeroes them first.
+ * exchange_zeroed_on_entry = 1 declares that the caller would have zeroed them:
+ * the kernel then writes every entry and never reads them (fewer bytes). */
+typedef struct p3x_infrastructure {
+  double dt;
+  int it, its, ite, kts, kte;
+  int predictNc;
+  int prescribedCCN;
+  const double* col_location;
+  int exchange_zeroed_on_entry;
+} p3x_infrastructure;
+
+/* ---- P3HistoryOnly (p3_functions.hpp:268-276) ---- */
+typedef struct p3x_history_only {
+  double *liq_ice_exchange, *vap_liq_exchange, *vap_ice_exchange;
+} p3x_history_only;
+
+/* ---- P3LookupTables contents, HOST arrays, row-major (p3_functions.hpp:279-290) ----
+ *   ice_table     [5][4][50][12]   collect_table [5][4][50][30][2] (log10 values)
+ *   vn/vm/revap   [300][10]        mu_r_table    [150]        dnu [16]            */
+typedef struct p3x_table_data {
+  double ice_table[5 * 4 * 50 * 12];
+  double collect_table[5 * 4 * 50 * 30 * 2];
+  double vn_table[300 * 10];
+  double vm_table[300 * 10];
+  double revap_table[300 * 10];
+  double mu_r_table[150];
+  double dnu[16];
+} p3x_table_data;
+
+/* ---- launch options (no reference counterpart: how the step runs on MI355X) ---- */
+enum { P3X_FP64 = 0, P3X_MIXED = 1 };
+typedef struct p3x_launch {
+  int precision;           /* P3X_FP64 (drop-in) or P3X_MIXED (FP32 bulk rates)  */
+  int ld;                  /* row stride of the level-major arrays; 0 means nj    */
+  int columns_per_block;   /* 0 = tuned default; else 16, 32 or 64                */
+  int threads_per_block;   /* 0 = tuned default; else 128, 256 or 512             */
+  void* stream;            /* hipStream_t; NULL = default stream                  */
+  int async;               /* 1: return after enqueueing (status from p3x_sync)   */
+  /* optional device arrays for the integer diagnostics (NULL = off):
+   *   col_status [nj]        0 skipped after part 1, 1 after part 2, 2 full
+   *   sed_substeps [3*nj]    CFL substeps of cloud, rain, ice sedimentation
+   *   ice_table_idx [nk*ld]  part-3 ice-table cell jj*10000+ii*100+i or -1      */
+  int* col_status;
+  int* sed_substeps;
+  int* ice_table_idx;
+} p3x_launch;
+
+typedef struct p3x_context p3x_context;
+
+/* Fill rt with the P3Runtime defaults (p3_functions.hpp:114-133). */
+void p3x_runtime_defaults(p3x_runtime* rt);
+/* Fill launch options with defaults (FP64, tuned block shape, synchronous). */
+void p3x_launch_defaults(p3x_launch* opts);
+
+/* Host: read the ice lookup table text (reader of p3_table_ice_impl.hpp:16-78,
+ * including the "VERSION 4.1.1" check) and build the rain tables (the
+ * reference generates them with micro_p3.F90 p3_init_b, :218-341) and dnu
+ * (p3_table3_impl.hpp:120-135). */
+int p3x_tables_load(const char* ice_table_file, p3x_table_data* out);
+
+/* Create a context on the current HIP device: uploads the tables. */
+int p3x_init(const p3x_table_data* tables, p3x_context** ctx);
+void p3x_finalize(p3x_context* ctx);
+
+/* One P3 step over nj columns x nk levels (p3_functions.hpp:1356-1369).
+ * elapsed_us (optional) receives the device time of the step (HIP events),
+ * the counterpart of p3_main's returned microseconds (p3_main_impl.hpp:328-330). */
+int p3x_main(p3x_context* ctx, const p3x_runtime* runtime,
+             const p3x_prognostic_state* prognostic_state,
+             const p3x_diagnostic_inputs* diagnostic_inputs,
+             const p3x_diagnostic_outputs* diagnostic_outputs,
+             const p3x_infrastructure* infrastructure,
+             const p3x_history_only* history_only,
+             int nj, int nk, const p3x_launch* opts, double* elapsed_us);
+
+/* Wait for the context's enqueued work and return its status. */
+int p3x_sync(p3x_context* ctx);
+
+const char* p3x_strerror(int status);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* P3X_H */

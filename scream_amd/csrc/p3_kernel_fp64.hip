@@ -1,0 +1,7 @@
+// FP64 drop-in variants of the fused P3 kernel.
+#include "p3_launch.hpp"
+namespace p3d {
+#define REAL double
+#define LAUNCH_NAME launch_p3_fp64
+#include "p3_kernel_variants.inc"
+}  // namespace p3d

@@ -1,0 +1,330 @@
+// p3_physics.hpp -- P3 per-level physics as CDNA4 device functions.
+//
+// Everything here is a pointwise function of one (column, level) point; the
+// kernel in p3_kernel.hip arranges the points.  The rate functions are
+// templated on the arithmetic type R of the bulk process rates:
+//   R = double : the FP64 drop-in; expressions keep the reference's operand
+//                order so results follow the reference's rounding.
+//   R = float  : the mixed form; saturation, the conservation limiters
+//                (water, number, supersaturation), the prognostic updates and
+//                every small difference of large FP64 quantities (qv - qsat,
+//                T - Tmelt, qv - qv_prev) stay FP64.
+// Function-by-function citations are to
+//   components/eamxx/src/physics/p3/impl/*_impl.hpp             (P3I)
+//   components/eamxx/src/physics/share/physics_saturation_impl.hpp (PSH)
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "p3_constants.hpp"
+
+namespace p3d {
+
+using namespace p3c;
+
+// ---------------------------------------------------------------- math shims
+// One spelling per precision so that the FP64 form calls exactly the double
+// routine and the mixed form exactly the float one.
+__device__ __forceinline__ double m_exp(double x) { return ::exp(x); }
+__device__ __forceinline__ float m_exp(float x) { return ::expf(x); }
+__device__ __forceinline__ double m_log(double x) { return ::log(x); }
+__device__ __forceinline__ float m_log(float x) { return ::logf(x); }
+__device__ __forceinline__ double m_log10(double x) { return ::log10(x); }
+__device__ __forceinline__ float m_log10(float x) { return ::log10f(x); }
+__device__ __forceinline__ double m_pow(double x, double y) { return ::pow(x, y); }
+__device__ __forceinline__ float m_pow(float x, float y) { return ::powf(x, y); }
+__device__ __forceinline__ double m_cbrt(double x) { return ::cbrt(x); }
+__device__ __forceinline__ float m_cbrt(float x) { return ::cbrtf(x); }
+__device__ __forceinline__ double m_sqrt(double x) { return ::sqrt(x); }
+__device__ __forceinline__ float m_sqrt(float x) { return ::sqrtf(x); }
+__device__ __forceinline__ double m_tgamma(double x) { return ::tgamma(x); }
+__device__ __forceinline__ float m_tgamma(float x) { return ::tgammaf(x); }
+__device__ __forceinline__ double m_expm1(double x) { return ::expm1(x); }
+__device__ __forceinline__ float m_expm1(float x) { return ::expm1f(x); }
+__device__ __forceinline__ double m_fabs(double x) { return ::fabs(x); }
+__device__ __forceinline__ float m_fabs(float x) { return ::fabsf(x); }
+
+// ekat::impl::min/max on scalars (ternary form, argument order preserved)
+template <class T> __device__ __forceinline__ T vmin(T a, T b) { return a < b ? a : b; }
+template <class T> __device__ __forceinline__ T vmax(T a, T b) { return a > b ? a : b; }
+
+// Error bits reported through the device error word.
+enum : unsigned { ERR_BAD_TEMPERATURE = 1u, ERR_SUBSTEP_LIMIT = 2u };
+
+// ---------------------------------------------------------------- saturation (FP64 always)
+// MurphyKoop_svp, PSH:38-77 (with check_temperature, PSH:14-36, as a flag)
+__device__ __forceinline__ double mk_svp(double t, bool ice, unsigned& err) {
+  if (t <= 0 || isnan(t)) err |= ERR_BAD_TEMPERATURE;
+  if (ice && t < Tmelt) {
+    return ::exp(9.550426 - (5723.265 / t) + (3.53068 * ::log(t)) - (0.00728332 * t));
+  }
+  const double logt = ::log(t);
+  return ::exp(54.842763 - (6763.22 / t) - (4.210 * logt) + (0.000367 * t) +
+               (::tanh(0.0415 * (t - 218.8)) *
+                (53.878 - (1331.22 / t) - (9.44523 * logt) + 0.014025 * t)));
+}
+// qv_sat_dry, PSH:127-157
+__device__ __forceinline__ double qv_sat_dry(double t, double p, bool ice, unsigned& err) {
+  const double e = mk_svp(t, ice, err);
+  return ep_2 * e / vmax(p, 1.e-3);
+}
+
+// ---------------------------------------------------------------- runtime options
+struct Runtime {  // P3Runtime, components/eamxx/src/physics/p3/p3_functions.hpp:112-158
+  double max_total_ni, autoconversion_prefactor, autoconversion_qc_exponent,
+      autoconversion_nc_exponent, autoconversion_radius, accretion_prefactor,
+      accretion_qc_exponent, accretion_qr_exponent, rain_selfcollection_prefactor,
+      rain_selfcollection_breakup_diameter, constant_mu_rain, spa_ccn_to_nc_factor,
+      cldliq_to_ice_collection_factor, rain_to_ice_collection_factor, min_rime_rho,
+      max_rime_rho, immersion_freezing_exponent, deposition_nucleation_exponent,
+      ice_sedimentation_factor;
+  int do_ice_production;
+};
+
+// ---------------------------------------------------------------- tables
+struct Tables {
+  const double* ice;      // [5][4][50][12]
+  const double* coll;     // [5][4][50][30][2]  (log10 values)
+  const double* vn;       // [300][10]
+  const double* vm;       // [300][10]
+  const double* revap;    // [300][10]
+};
+
+template <class R> struct TabIce { int dumi, dumii, dumjj; R dum1, dum4, dum5; };
+template <class R> struct TabRain { int dumj; R dum3; };
+template <class R> struct Tab3 { int dumii, dumjj; R rdumii, rdumjj; };
+
+// lookup_ice, P3I/p3_table_ice_impl.hpp:80-138
+template <class R>
+__device__ __forceinline__ TabIce<R> lookup_ice(R qi, R ni, R qm, R rhop) {
+  TabIce<R> t;
+  t.dum1 = (m_log10(qi / ni) + R(18)) * R(lookup_table_1a_dum1_c) - R(10);
+  t.dumi = (int)t.dum1;
+  t.dum1 = vmin(t.dum1, R(isize));
+  t.dum1 = vmax(t.dum1, R(1.));
+  t.dumi = vmax(1, t.dumi);
+  t.dumi = vmin(isize - 1, t.dumi);
+  t.dum4 = (qm / qi) * R(3) + R(1);
+  t.dumii = (int)t.dum4;
+  t.dum4 = vmin(t.dum4, R(rimsize));
+  t.dum4 = vmax(t.dum4, R(1.));
+  t.dumii = vmax(1, t.dumii);
+  t.dumii = vmin(rimsize - 1, t.dumii);
+  if (rhop <= R(650)) t.dum5 = (rhop - R(50)) * R(0.005) + R(1);
+  else t.dum5 = (rhop - R(650)) * R(0.004) + R(4);
+  t.dumjj = (int)t.dum5;
+  t.dum5 = vmin(t.dum5, R(densize));
+  t.dum5 = vmax(t.dum5, R(1.));
+  t.dumjj = vmax(1, t.dumjj);
+  t.dumjj = vmin(densize - 1, t.dumjj);
+  t.dumi -= 1;
+  t.dumjj -= 1;
+  t.dumii -= 1;
+  return t;
+}
+
+// lookup_rain, P3I/p3_table_ice_impl.hpp:140-173
+template <class R>
+__device__ __forceinline__ TabRain<R> lookup_rain(R qr, R nr) {
+  TabRain<R> t;
+  const bool gt_small = qr > R(QSMALL) && nr > R(0.0);
+  R dumlr = R(1);
+  if (gt_small) dumlr = m_cbrt(qr / (R(Pi) * R(RHO_H2O) * nr));
+  t.dum3 = (m_log10(R(1) * dumlr) + R(5)) * R(10.70415);
+  t.dumj = (int)t.dum3;
+  t.dum3 = vmin(t.dum3, R(rcollsize));
+  t.dum3 = vmax(t.dum3, R(1));
+  t.dumj = vmax(1, t.dumj);
+  t.dumj = vmin(rcollsize - 1, t.dumj);
+  if (!gt_small) { t.dumj = 1; t.dum3 = R(1); }
+  t.dumj -= 1;
+  return t;
+}
+
+__device__ __forceinline__ int ice_off(int jj, int ii, int i, int j) {
+  return (((jj * rimsize + ii) * isize + i) * ice_table_size) + j;
+}
+__device__ __forceinline__ int coll_off(int jj, int ii, int i, int j, int k) {
+  return ((((jj * rimsize + ii) * isize + i) * rcollsize + j) * collect_table_size) + k;
+}
+
+// apply_table_ice, P3I/p3_table_ice_impl.hpp:175-217
+template <class R>
+__device__ __forceinline__ R apply_table_ice(const double* __restrict__ tab, int idx, const TabIce<R>& t) {
+  const int jj = t.dumjj, ii = t.dumii, i = t.dumi;
+  const R w1 = (t.dum1 - R(i) - R(1));
+  const R w4 = (t.dum4 - R(ii) - R(1));
+  auto T = [&](int a, int b, int c) { return R(tab[ice_off(a, b, c, idx)]); };
+  R iproc1 = T(jj, ii, i) + w1 * (T(jj, ii, i + 1) - T(jj, ii, i));
+  R gproc1 = T(jj, ii + 1, i) + w1 * (T(jj, ii + 1, i + 1) - T(jj, ii + 1, i));
+  const R tmp1 = iproc1 + w4 * (gproc1 - iproc1);
+  iproc1 = T(jj + 1, ii, i) + w1 * (T(jj + 1, ii, i + 1) - T(jj + 1, ii, i));
+  gproc1 = T(jj + 1, ii + 1, i) + w1 * (T(jj + 1, ii + 1, i + 1) - T(jj + 1, ii + 1, i));
+  const R tmp2 = iproc1 + w4 * (gproc1 - iproc1);
+  return tmp1 + (t.dum5 - R(jj) - R(1)) * (tmp2 - tmp1);
+}
+
+// apply_table_coll, P3I/p3_table_ice_impl.hpp:219-297
+template <class R>
+__device__ __forceinline__ R apply_table_coll(const double* __restrict__ tab, int idx, const TabIce<R>& ti,
+                                              const TabRain<R>& tr) {
+  const int jj = ti.dumjj, ii = ti.dumii, i = ti.dumi, j = tr.dumj;
+  const R w1 = (ti.dum1 - R(i) - R(1));
+  const R w3 = (tr.dum3 - R(j) - R(1));
+  const R w4 = (ti.dum4 - R(ii) - R(1));
+  auto T = [&](int a, int b, int c, int d) { return R(tab[coll_off(a, b, c, d, idx)]); };
+  R dproc1 = T(jj, ii, i, j) + w1 * (T(jj, ii, i + 1, j) - T(jj, ii, i, j));
+  R dproc2 = T(jj, ii, i, j + 1) + w1 * (T(jj, ii, i + 1, j + 1) - T(jj, ii, i, j + 1));
+  R iproc1 = dproc1 + w3 * (dproc2 - dproc1);
+  dproc1 = T(jj, ii + 1, i, j) + w1 * (T(jj, ii + 1, i + 1, j) - T(jj, ii + 1, i, j));
+  dproc2 = T(jj, ii + 1, i, j + 1) + w1 * (T(jj, ii + 1, i + 1, j + 1) - T(jj, ii + 1, i, j + 1));
+  R gproc1 = dproc1 + w3 * (dproc2 - dproc1);
+  const R tmp1 = iproc1 + w4 * (gproc1 - iproc1);
+  dproc1 = T(jj + 1, ii, i, j) + w1 * (T(jj + 1, ii, i + 1, j) - T(jj + 1, ii, i, j));
+  dproc2 = T(jj + 1, ii, i, j + 1) + w1 * (T(jj + 1, ii, i + 1, j + 1) - T(jj + 1, ii, i, j + 1));
+  iproc1 = dproc1 + w3 * (dproc2 - dproc1);
+  dproc1 = T(jj + 1, ii + 1, i, j) + w1 * (T(jj + 1, ii + 1, i + 1, j) - T(jj + 1, ii + 1, i, j));
+  dproc2 = T(jj + 1, ii + 1, i, j + 1) + w1 * (T(jj + 1, ii + 1, i + 1, j + 1) - T(jj + 1, ii + 1, i, j + 1));
+  gproc1 = dproc1 + w3 * (dproc2 - dproc1);
+  const R tmp2 = iproc1 + w4 * (gproc1 - iproc1);
+  return tmp1 + (ti.dum5 - R(jj) - R(1)) * (tmp2 - tmp1);
+}
+
+// lookup (Table3), P3I/p3_table3_impl.hpp:14-62
+template <class R>
+__device__ __forceinline__ Tab3<R> lookup3(R mu_r, R lamr) {
+  Tab3<R> t;
+  const R dum1 = (mu_r + R(1)) / lamr;
+  if (dum1 <= R(195.e-6)) {
+    const R inv_dum3 = R(0.1);
+    R rdumii = (dum1 * R(1.e6) + R(5)) * inv_dum3;
+    rdumii = vmax(rdumii, R(1.0));
+    rdumii = vmin(rdumii, R(20.0));
+    int dumii = (int)rdumii;
+    dumii = vmax(dumii, 1);
+    dumii = vmin(dumii, 20);
+    t.rdumii = rdumii;
+    t.dumii = dumii;
+  } else {
+    const R inv_dum3 = R(THIRD) * R(0.1);
+    R rdumii = (dum1 * R(1.e+6) - R(195)) * inv_dum3 + R(20);
+    rdumii = vmax(rdumii, R(20.0));
+    rdumii = vmin(rdumii, R(300.0));
+    int dumii = (int)rdumii;
+    dumii = vmax(dumii, 20);
+    dumii = vmin(dumii, 299);
+    t.rdumii = rdumii;
+    t.dumii = dumii;
+  }
+  R rdumjj = mu_r + R(1);
+  rdumjj = vmax(rdumjj, R(1));
+  rdumjj = vmin(rdumjj, R(10));
+  int dumjj = (int)rdumjj;
+  dumjj = vmax(dumjj, 1);
+  dumjj = vmin(dumjj, 9);
+  t.rdumjj = rdumjj;
+  t.dumjj = dumjj;
+  return t;
+}
+
+// apply_table, P3I/p3_table3_impl.hpp:64-83 ([300][10] table)
+template <class R>
+__device__ __forceinline__ R apply_table3(const double* __restrict__ table, const Tab3<R>& t) {
+  const R rdumii_m_dumii = t.rdumii - R(t.dumii);
+  const R t_im1_jm1 = R(table[(t.dumii - 1) * VTABLE_DIM1 + (t.dumjj - 1)]);
+  const R dum1 = (t_im1_jm1 + rdumii_m_dumii * (R(table[t.dumii * VTABLE_DIM1 + (t.dumjj - 1)]) - t_im1_jm1));
+  const R t_im1_j = R(table[(t.dumii - 1) * VTABLE_DIM1 + t.dumjj]);
+  const R dum2 = (t_im1_j + rdumii_m_dumii * (R(table[t.dumii * VTABLE_DIM1 + t.dumjj]) - t_im1_j));
+  return dum1 + (t.rdumjj - R(t.dumjj)) * (dum2 - dum1);
+}
+
+// ---------------------------------------------------------------- size distributions
+// get_cloud_dsd2, P3I/p3_dsd2_impl.hpp:14-71 (iparam = 3, nu stays 0)
+template <class R>
+__device__ __forceinline__ void get_cloud_dsd2(R qc, R& nc, R& mu_c, R rho, R& lamc, R& cdist, R& cdist1) {
+  lamc = R(0); cdist = R(0); cdist1 = R(0); mu_c = R(0);
+  if (qc >= R(QSMALL)) {
+    nc = vmax(nc, R(NSMALL));
+    R m = R(0.0005714) * (nc * R(1.e-6) * rho) + R(0.2714);
+    m = R(1) / (m * m) - R(1);
+    m = vmax(m, R(2));
+    m = vmin(m, R(15));
+    mu_c = m;
+    lamc = m_cbrt(R(CONS1) * nc * (m + R(3)) * (m + R(2)) * (m + R(1)) / qc);
+    const R lammin = (m + R(1)) * R(2.5e+4);
+    const R lammax = (m + R(1)) * R(1.e+6);
+    const bool lt = lamc < lammin, gt = lamc > lammax;
+    if (lt) lamc = lammin;
+    if (gt) lamc = lammax;
+    if (lt || gt)
+      nc = R(6) * (lamc * lamc * lamc) * qc / (R(Pi) * R(RHO_H2O) * (m + R(3)) * (m + R(2)) * (m + R(1)));
+    cdist = nc * (m + R(1)) / lamc;
+    cdist1 = nc / m_tgamma(m + R(1));
+  }
+}
+
+// get_rain_dsd2, P3I/p3_dsd2_impl.hpp:73-126
+template <class R>
+__device__ __forceinline__ void get_rain_dsd2(const Runtime& rt, R qr, R& nr, R& mu_r, R& lamr) {
+  lamr = R(0);
+  if (qr >= R(QSMALL)) {
+    const R nr_lim = vmax(nr, R(NSMALL));
+    mu_r = R(rt.constant_mu_rain);
+    const R f = R(CONS1) * (mu_r + R(3)) * (mu_r + R(2)) * (mu_r + R(1));
+    lamr = m_cbrt(f * nr_lim / qr);
+    const R lammax = (mu_r + R(1.)) * R(1.e+5);
+    const R lammin = (mu_r + R(1.)) * R(500);
+    const bool lt = lamr < lammin, gt = lamr > lammax;
+    nr = nr_lim;
+    if (lt || gt) {
+      if (lt) lamr = lammin;
+      if (gt) lamr = lammax;
+      nr = lamr * lamr * lamr * qr / f;
+    }
+  }
+}
+
+// calc_bulk_rho_rime, P3I/p3_ice_sed_impl.hpp:14-64
+template <class R>
+__device__ __forceinline__ R calc_bulk_rho_rime(const Runtime& rt, R qi_tot, R& qi_rim, R& bi_rim) {
+  R rho_rime = R(0);
+  const bool gt = bi_rim >= R(BSMALL), lt = bi_rim < R(BSMALL);
+  if (gt) rho_rime = qi_rim / bi_rim;
+  const bool lt_min = rho_rime < R(rt.min_rime_rho), gt_max = rho_rime > R(rt.max_rime_rho);
+  if (gt && lt_min) rho_rime = R(rt.min_rime_rho);
+  if (gt && gt_max) rho_rime = R(rt.max_rime_rho);
+  if (gt && (gt_max || lt_min)) bi_rim = qi_rim / rho_rime;
+  if (lt) { qi_rim = R(0); bi_rim = R(0); rho_rime = R(0); }
+  if (qi_rim > qi_tot && rho_rime > R(0)) {
+    qi_rim = qi_tot;
+    bi_rim = qi_rim / rho_rime;
+  }
+  if (qi_rim < R(QSMALL)) { qi_rim = R(0); bi_rim = R(0); }
+  return rho_rime;
+}
+
+// calculate_incloud_mixingratios, P3I/p3_incloud_mixingratios_impl.hpp:9-67
+struct Incld { double qc, qr, qi, qm, nc, nr, ni, bm; };
+__device__ __forceinline__ Incld incloud(double qc, double qr, double qi, double qm, double nc, double nr,
+                                         double ni, double bm, double icfl, double icfi, double icfr) {
+  Incld o;
+  if (qc >= QSMALL) { o.qc = qc * icfl; o.nc = vmax(nc * icfl, 0.0); } else { o.qc = 0; o.nc = 0; }
+  if (qi >= QSMALL) { o.qi = qi * icfi; o.ni = vmax(ni * icfi, 0.0); } else { o.qi = 0; o.ni = 0; }
+  if (qi >= QSMALL && qm >= QSMALL) { o.qm = qm * icfi; o.bm = vmax(bm * icfl, 0.0); } else { o.qm = 0; o.bm = 0; }
+  if (qr >= QSMALL) { o.qr = qr * icfr; o.nr = vmax(nr * icfr, 0.0); } else { o.qr = 0; o.nr = 0; }
+  if (o.qc > incloud_limit || o.qi > incloud_limit || o.qr > precip_limit || o.bm > incloud_limit) {
+    o.qc = vmin(o.qc, incloud_limit);
+    o.qi = vmin(o.qi, incloud_limit);
+    o.bm = vmin(o.bm, incloud_limit);
+    o.qr = vmin(o.qr, precip_limit);
+  }
+  return o;
+}
+
+// impose_max_total_ni, P3I/p3_impose_max_total_ni_impl.hpp:9-26
+template <class R>
+__device__ __forceinline__ void impose_max_total_ni(R& ni, R max_total_ni, R inv_rho) {
+  if (ni >= R(1e-20)) ni = ni * vmin(max_total_ni * inv_rho / ni, R(1));
+}
+
+}  // namespace p3d

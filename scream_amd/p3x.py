@@ -1,0 +1,202 @@
+"""ctypes binding of the P3 C ABI (include/p3x.h) for Python callers.
+
+This is the Python side of the drop-in boundary: it passes device pointers
+(torch CUDA tensors or raw addresses) straight to ``p3x_main``; no compute
+happens here.  The library must be built (``python -m scream_amd.build``);
+there is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libp3x.so")
+ICE_TABLE = os.path.join(HERE, "data", "p3_lookup_table_1.dat-v4.1.1")
+
+P3X_OK, P3X_ERR_ARG, P3X_ERR_TABLE_FILE, P3X_ERR_TABLE_VERSION = 0, 1, 2, 3
+P3X_ERR_DEVICE, P3X_ERR_TEMPERATURE, P3X_ERR_NO_MEMORY = 4, 5, 6
+P3X_FP64, P3X_MIXED = 0, 1
+
+RUNTIME_KEYS = (
+    "max_total_ni", "autoconversion_prefactor", "autoconversion_qc_exponent",
+    "autoconversion_nc_exponent", "autoconversion_radius", "accretion_prefactor",
+    "accretion_qc_exponent", "accretion_qr_exponent", "rain_selfcollection_prefactor",
+    "rain_selfcollection_breakup_diameter", "constant_mu_rain", "spa_ccn_to_nc_factor",
+    "cldliq_to_ice_collection_factor", "rain_to_ice_collection_factor", "min_rime_rho",
+    "max_rime_rho", "immersion_freezing_exponent", "deposition_nucleation_exponent",
+    "ice_sedimentation_factor")
+STATE = ("qc", "nc", "qr", "nr", "qi", "qm", "ni", "bm", "qv", "th")
+INPUTS = ("nc_nuceat_tend", "nccn", "ni_activated", "inv_qc_relvar", "cld_frac_i",
+          "cld_frac_l", "cld_frac_r", "pres", "dz", "dpres", "inv_exner", "qv_prev", "t_prev")
+OUTS = ("qv2qi_depos_tend", "precip_liq_surf", "precip_ice_surf", "diag_eff_radius_qc",
+        "diag_eff_radius_qi", "diag_eff_radius_qr", "rho_qi", "precip_liq_flux",
+        "precip_ice_flux", "precip_total_tend", "nevapr")
+HIST = ("liq_ice_exchange", "vap_liq_exchange", "vap_ice_exchange")
+ALL_FIELDS = STATE + INPUTS + OUTS + HIST
+
+# Every symbol include/p3x.h declares.
+EXPORTS = ("p3x_runtime_defaults", "p3x_launch_defaults", "p3x_tables_load", "p3x_init",
+           "p3x_finalize", "p3x_main", "p3x_sync", "p3x_strerror")
+
+
+class Runtime(C.Structure):
+    _fields_ = [(k, C.c_double) for k in RUNTIME_KEYS] + [("do_ice_production", C.c_int)]
+
+
+class PrognosticState(C.Structure):
+    _fields_ = [(k, C.c_void_p) for k in STATE]
+
+
+class DiagnosticInputs(C.Structure):
+    _fields_ = [(k, C.c_void_p) for k in INPUTS]
+
+
+class DiagnosticOutputs(C.Structure):
+    _fields_ = [(k, C.c_void_p) for k in OUTS]
+
+
+class Infrastructure(C.Structure):
+    _fields_ = [("dt", C.c_double), ("it", C.c_int), ("its", C.c_int), ("ite", C.c_int),
+                ("kts", C.c_int), ("kte", C.c_int), ("predictNc", C.c_int),
+                ("prescribedCCN", C.c_int), ("col_location", C.c_void_p),
+                ("exchange_zeroed_on_entry", C.c_int)]
+
+
+class HistoryOnly(C.Structure):
+    _fields_ = [(k, C.c_void_p) for k in HIST]
+
+
+class TableData(C.Structure):
+    _fields_ = [("ice_table", C.c_double * 12000), ("collect_table", C.c_double * 60000),
+                ("vn_table", C.c_double * 3000), ("vm_table", C.c_double * 3000),
+                ("revap_table", C.c_double * 3000), ("mu_r_table", C.c_double * 150),
+                ("dnu", C.c_double * 16)]
+
+
+class Launch(C.Structure):
+    _fields_ = [("precision", C.c_int), ("ld", C.c_int), ("columns_per_block", C.c_int),
+                ("threads_per_block", C.c_int), ("stream", C.c_void_p), ("async_", C.c_int),
+                ("col_status", C.c_void_p), ("sed_substeps", C.c_void_p),
+                ("ice_table_idx", C.c_void_p)]
+
+
+class P3Error(RuntimeError):
+    def __init__(self, status: int, what: str = ""):
+        self.status = status
+        super().__init__(f"{what}: p3x status {status} ({strerror(status)})")
+
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """The compiled library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m scream_amd.build`")
+        lib = C.CDLL(LIB_PATH)
+        lib.p3x_runtime_defaults.argtypes = [C.POINTER(Runtime)]
+        lib.p3x_launch_defaults.argtypes = [C.POINTER(Launch)]
+        lib.p3x_tables_load.argtypes = [C.c_char_p, C.POINTER(TableData)]
+        lib.p3x_tables_load.restype = C.c_int
+        lib.p3x_init.argtypes = [C.POINTER(TableData), C.POINTER(C.c_void_p)]
+        lib.p3x_init.restype = C.c_int
+        lib.p3x_finalize.argtypes = [C.c_void_p]
+        lib.p3x_main.argtypes = [C.c_void_p, C.POINTER(Runtime), C.POINTER(PrognosticState),
+                                 C.POINTER(DiagnosticInputs), C.POINTER(DiagnosticOutputs),
+                                 C.POINTER(Infrastructure), C.POINTER(HistoryOnly), C.c_int,
+                                 C.c_int, C.POINTER(Launch), C.POINTER(C.c_double)]
+        lib.p3x_main.restype = C.c_int
+        lib.p3x_sync.argtypes = [C.c_void_p]
+        lib.p3x_sync.restype = C.c_int
+        lib.p3x_strerror.argtypes = [C.c_int]
+        lib.p3x_strerror.restype = C.c_char_p
+        _lib = lib
+    return _lib
+
+
+def strerror(status: int) -> str:
+    return load().p3x_strerror(int(status)).decode()
+
+
+def runtime_defaults(**overrides) -> Runtime:
+    rt = Runtime()
+    load().p3x_runtime_defaults(C.byref(rt))
+    for k, v in overrides.items():
+        setattr(rt, k, v)
+    return rt
+
+
+def tables_load(path: str = ICE_TABLE) -> TableData:
+    t = TableData()
+    st = load().p3x_tables_load(path.encode(), C.byref(t))
+    if st != P3X_OK:
+        raise P3Error(st, f"p3x_tables_load({path})")
+    return t
+
+
+def _addr(x) -> int:
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return x
+    return x.data_ptr()  # torch tensor on the device
+
+
+class Context:
+    """A p3x context on the current HIP device (tables resident in HBM)."""
+
+    def __init__(self, tables: TableData | None = None):
+        self._lib = load()
+        self.tables = tables if tables is not None else tables_load()
+        h = C.c_void_p()
+        st = self._lib.p3x_init(C.byref(self.tables), C.byref(h))
+        if st != P3X_OK:
+            raise P3Error(st, "p3x_init")
+        self.handle = h
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self._lib.p3x_finalize(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def main(self, f: dict, nj: int, nk: int, dt: float, predict_nc=True, prescribed_ccn=False,
+             runtime: Runtime | None = None, precision: int = P3X_FP64, ld: int = 0,
+             columns_per_block: int = 0, threads_per_block: int = 0, stream=None,
+             async_: bool = False, debug: dict | None = None, exchange_zeroed=False,
+             check: bool = True) -> float:
+        """One p3x_main call; ``f`` maps every field name to a device array.
+        Returns the step's device time in microseconds (0 when async)."""
+        rt = runtime if runtime is not None else runtime_defaults()
+        ps = PrognosticState(*[_addr(f[k]) for k in STATE])
+        di = DiagnosticInputs(*[_addr(f[k]) for k in INPUTS])
+        do = DiagnosticOutputs(*[_addr(f[k]) for k in OUTS])
+        hi = HistoryOnly(*[_addr(f[k]) for k in HIST])
+        inf = Infrastructure(dt=float(dt), it=1, its=0, ite=nj - 1, kts=0, kte=nk - 1,
+                             predictNc=int(bool(predict_nc)),
+                             prescribedCCN=int(bool(prescribed_ccn)), col_location=None,
+                             exchange_zeroed_on_entry=int(bool(exchange_zeroed)))
+        dbg = debug or {}
+        lo = Launch(precision=int(precision), ld=int(ld), columns_per_block=int(columns_per_block),
+                    threads_per_block=int(threads_per_block), stream=stream,
+                    async_=int(bool(async_)), col_status=_addr(dbg.get("col_status")),
+                    sed_substeps=_addr(dbg.get("sed_substeps")),
+                    ice_table_idx=_addr(dbg.get("ice_table_idx")))
+        us = C.c_double(0.0)
+        st = self._lib.p3x_main(self.handle, C.byref(rt), C.byref(ps), C.byref(di), C.byref(do),
+                                C.byref(inf), C.byref(hi), int(nj), int(nk), C.byref(lo),
+                                C.byref(us))
+        if check and st != P3X_OK:
+            raise P3Error(st, "p3x_main")
+        return us.value if st == P3X_OK else float("nan")
+
+    def sync(self) -> int:
+        return self._lib.p3x_sync(self.handle)

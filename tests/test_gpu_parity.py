@@ -1,0 +1,87 @@
+"""Parity of the HIP p3_main with the FP64 oracle (run through the C ABI).
+
+Bar (FP64 drop-in): integer outputs bit-exact (column status, CFL substep
+counts, part-3 ice-table cells); every floating-point field within 1e-10
+pointwise relative error of the oracle on identical inputs and tables; the
+column total-water budget closing to roundoff.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from scream_amd import p3x
+from scream_amd.inputs import make_columns
+
+from p3_compare import field_errors, run_gpu, water_budget
+
+pytestmark = pytest.mark.gpu
+
+RTOL_FP64 = 1e-10     # pointwise relative tolerance, FP64 form vs oracle
+BUDGET_RTOL = 1e-12   # column water budget closure (relative to the column total)
+
+
+@pytest.fixture(scope="module")
+def ctx(host_tables):
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    c = p3x.Context(host_tables)
+    yield c
+    c.close()
+
+
+def _check_fp64(got, gdbg, want, wdbg, label):
+    errs = field_errors(got, want)
+    bad = {k: v for k, v in errs.items() if not v <= RTOL_FP64}
+    worst = max(errs.items(), key=lambda kv: kv[1])
+    print(f"[{label}] worst field {worst[0]} rel={worst[1]:.3e}")
+    assert not bad, f"[{label}] fields beyond {RTOL_FP64}: {bad}"
+    np.testing.assert_array_equal(gdbg["col_status"], wdbg["col_status"], err_msg=label)
+    np.testing.assert_array_equal(gdbg["sed_substeps"], wdbg["sed_substeps"], err_msg=label)
+    np.testing.assert_array_equal(gdbg["ice_table_idx"], wdbg["ice_table_idx"], err_msg=label)
+
+
+CASES = [
+    # ncol, nlev, dt, predictNc, prescribedCCN, seed
+    (256, 72, 300.0, True, False, 11),     # the CPU reference configuration
+    (300, 72, 1800.0, True, False, 12),
+    (257, 128, 100.0, True, False, 13),
+    (200, 72, 1800.0, False, False, 14),
+    (200, 72, 300.0, True, True, 15),
+    (200, 128, 1800.0, False, True, 16),
+]
+
+
+@pytest.mark.parametrize("ncol,nlev,dt,pnc,pccn,seed", CASES)
+def test_fp64_matches_oracle(ctx, oracle_tables, ncol, nlev, dt, pnc, pccn, seed):
+    inp = make_columns(ncol, nlev, seed=seed)
+    want, wdbg, nerr = oracle.run_oracle(inp, oracle_tables, dt, pnc, pccn)
+    assert nerr == 0
+    got, gdbg, _ = run_gpu(ctx, inp, dt, pnc, pccn)
+    _check_fp64(got, gdbg, want, wdbg, f"{ncol}x{nlev} dt={dt} pnc={pnc} pccn={pccn}")
+    before, after = water_budget(inp, got, dt)
+    assert np.all(np.abs(after - before) <= BUDGET_RTOL * np.abs(before))
+
+
+@pytest.mark.parametrize("cb,nt", [(16, 128), (16, 256), (32, 256), (64, 256), (32, 512), (64, 512)])
+def test_fp64_block_shapes_agree(ctx, oracle_tables, cb, nt):
+    """Every compiled block shape gives the same (oracle-matching) answer."""
+    inp = make_columns(333, 72, seed=21)
+    want, wdbg, _ = oracle.run_oracle(inp, oracle_tables, 1800.0, True, False)
+    got, gdbg, _ = run_gpu(ctx, inp, 1800.0, True, False, columns_per_block=cb, threads_per_block=nt)
+    _check_fp64(got, gdbg, want, wdbg, f"block {cb}x{nt}")
+
+
+def test_fp64_full_size_g64(ctx, oracle_tables):
+    """ne30pg2-sized physics grid: 21600 columns x 72 levels, dt 1800 s."""
+    inp = make_columns(21600, 72, seed=1)
+    want, wdbg, nerr = oracle.run_oracle(inp, oracle_tables, 1800.0, True, False)
+    got, gdbg, _ = run_gpu(ctx, inp, 1800.0, True, False)
+    _check_fp64(got, gdbg, want, wdbg, "G64 21600x72")
+
+
+def test_fp64_full_size_mix(ctx, oracle_tables):
+    """One GPU's share of a high-resolution grid: 16384 x 128, dt 100 s (FP64 form)."""
+    inp = make_columns(16384, 128, seed=2)
+    want, wdbg, nerr = oracle.run_oracle(inp, oracle_tables, 100.0, True, False)
+    got, gdbg, _ = run_gpu(ctx, inp, 100.0, True, False)
+    _check_fp64(got, gdbg, want, wdbg, "MIX 16384x128 fp64")
