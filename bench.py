@@ -1,0 +1,201 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X P3 microphysics step (p3_main drop-in).
+
+A "step" is one p3_main call over one batch of synthetic columns, with the
+inputs already resident in HBM.  Headline workload: the FP64 drop-in on the
+ne30pg2-sized physics grid, 21600 columns x 72 levels, dt = 1800 s.  Before
+every step the prognostic state is restored from a pristine device copy (so
+every step sees the same work); that copy is inside the timed region.
+
+Also reported (keys "forms"): the FP64 and mixed-precision forms on one GPU's
+share of a high-resolution grid (16384 x 128, dt = 100 s) with the mixed
+form's per-field error against the FP64 result, the column-level roofline of
+the dominant kernel, and the oracle (CPU restatement) timed on this host.
+
+Prints ONE JSON line.
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "P3 column-levels/s, FP64 vs mixed at stated error; served p50/p99 step latency"
+BYTES_PER_COLUMN_LEVEL = 360.0   # 23 FP64 fields read + 22 written per column-level
+HBM_PEAK_GBS = 8000.0            # MI355X published HBM3E bandwidth
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-forms", action="store_true", help="skip the 16384x128 forms")
+    return ap.parse_args()
+
+
+class Workload:
+    """Device-resident fields for one column set, plus a pristine copy of the state."""
+
+    def __init__(self, ncol, nlev, seed):
+        import torch
+        from scream_amd import p3x
+        from scream_amd.inputs import make_columns
+        self.ncol, self.nlev = ncol, nlev
+        dev = torch.device("cuda:0")
+        inp = make_columns(ncol, nlev, seed=seed)
+        self.f = {k: torch.from_numpy(inp[k]).to(dev) for k in p3x.STATE + p3x.INPUTS}
+        for k in p3x.OUTS + p3x.HIST:
+            shape = (ncol,) if "surf" in k else ((nlev + 1, ncol) if "flux" in k else (nlev, ncol))
+            self.f[k] = torch.zeros(shape, dtype=torch.float64, device=dev)
+        self.pristine = {k: self.f[k].clone() for k in p3x.STATE}
+
+    def restore(self):
+        for k, v in self.pristine.items():
+            self.f[k].copy_(v)
+
+
+def time_steps(ctx, w, dt, precision, steps, warmup):
+    """Wall ms/step (sync-bracketed) and mean kernel us/step (HIP events on the kernel's stream)."""
+    import torch
+    stream = torch.cuda.current_stream()
+    sp = stream.cuda_stream
+    for _ in range(warmup):
+        w.restore()
+        ctx.main(w.f, w.ncol, w.nlev, dt, True, False, precision=precision, stream=sp,
+                 async_=True, exchange_zeroed=True)
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for e0, e1 in evs:
+        w.restore()
+        e0.record(stream)
+        ctx.main(w.f, w.ncol, w.nlev, dt, True, False, precision=precision, stream=sp,
+                 async_=True, exchange_zeroed=True)
+        e1.record(stream)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    st = ctx.sync()
+    if st != 0:
+        raise RuntimeError(f"p3x status {st}")
+    kern_us = [1000.0 * e0.elapsed_time(e1) for e0, e1 in evs]
+    return 1000.0 * (t1 - t0) / steps, statistics.mean(kern_us)
+
+
+def mixed_error(ctx, w, dt):
+    """Per-field max |mixed - fp64| / max|fp64| and RMS ratio on one step (outputs of the FP64 form
+    match the FP64 oracle to 1e-10, so this is the mixed form's error against the oracle)."""
+    import torch
+    from scream_amd import p3x
+    sp = torch.cuda.current_stream().cuda_stream
+    fields = p3x.STATE + p3x.OUTS + p3x.HIST
+    w.restore()
+    ctx.main(w.f, w.ncol, w.nlev, dt, True, False, precision=p3x.P3X_FP64, stream=sp, exchange_zeroed=True)
+    ref = {k: w.f[k].clone() for k in fields}
+    w.restore()
+    ctx.main(w.f, w.ncol, w.nlev, dt, True, False, precision=p3x.P3X_MIXED, stream=sp, exchange_zeroed=True)
+    out = {}
+    for k in fields:
+        d = (w.f[k] - ref[k]).abs()
+        scale = ref[k].abs().max().item()
+        rms_ref = ref[k].pow(2).mean().sqrt().item()
+        out[k] = {"max_rel_to_field_max": (d.max().item() / scale) if scale > 0 else 0.0,
+                  "rms_rel": ((d.pow(2).mean().sqrt().item() / rms_ref) if rms_ref > 0 else 0.0)}
+    return out
+
+
+def cpu_baseline():
+    """The oracle (FP64 C restatement, OpenMP over columns) on a bounded sample of the
+    headline workload: 8192 columns x 72 levels, dt 1800 s, median of 5 runs."""
+    import oracle
+    from scream_amd import p3x
+    from scream_amd.inputs import make_columns
+    import numpy as np
+    t = p3x.tables_load()
+    a = lambda x, sh: np.ctypeslib.as_array(x).reshape(sh).copy()
+    tabs = dict(ice=a(t.ice_table, (5, 4, 50, 12)), coll=a(t.collect_table, (5, 4, 50, 30, 2)),
+                vn=a(t.vn_table, (300, 10)), vm=a(t.vm_table, (300, 10)),
+                revap=a(t.revap_table, (300, 10)), mu_r=a(t.mu_r_table, (150,)), dnu=a(t.dnu, (16,)))
+    ncol, nlev = 8192, 72
+    inp = make_columns(ncol, nlev, seed=1)
+    nthreads = max(1, min(16, os.cpu_count() or 1))
+    times = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        oracle.run_oracle(inp, tabs, 1800.0, True, False, nthreads=nthreads)
+        times.append(time.perf_counter() - t0)
+    med = statistics.median(times)
+    return {"value": ncol * nlev / med, "unit": "column-levels/s", "cores": nthreads, "kind": "port",
+            "sample": f"oracle/p3_oracle.c FP64, {ncol}x{nlev} columns of the same generator, "
+                      f"dt 1800 s, median of 5 runs ({med:.3f} s each)"}
+
+
+def traffic_for(workload):
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        return json.load(fh).get(workload)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "P3 columns are independent, so N GPUs would each run this workload on their "
+              "own column range", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from scream_amd import p3x
+    assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    ctx = p3x.Context()
+
+    ncol, nlev, dt = 21600, 72, 1800.0
+    w = Workload(ncol, nlev, seed=1)
+    ms, kern_us = time_steps(ctx, w, dt, p3x.P3X_FP64, args.steps, args.warmup)
+    colev = ncol * nlev
+    value = colev / (ms / 1000.0)
+    achieved = BYTES_PER_COLUMN_LEVEL * colev / (kern_us * 1e-6) / 1e9
+    workload = f"p3_main FP64 drop-in, {ncol}x{nlev}, dt={dt:g} s"
+    line = {
+        "metric": METRIC, "value": value, "unit": "column-levels/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": workload, "ncol": ncol, "nlev": nlev, "dt": dt,
+                   "predictNc": True, "prescribedCCN": False, "kernel_us_per_step": kern_us},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_for("g64_fp64")},
+    }
+    del w
+    if not args.no_forms:
+        ncol2, nlev2, dt2 = 16384, 128, 100.0
+        w2 = Workload(ncol2, nlev2, seed=2)
+        forms = {}
+        for name, prec in (("fp64", p3x.P3X_FP64), ("mixed", p3x.P3X_MIXED)):
+            ms2, k2 = time_steps(ctx, w2, dt2, prec, max(5, args.steps // 2), args.warmup)
+            forms[name] = {"column_levels_per_s": ncol2 * nlev2 / (ms2 / 1000.0), "ms_per_step": ms2,
+                           "kernel_us_per_step": k2,
+                           "hbm_frac": BYTES_PER_COLUMN_LEVEL * ncol2 * nlev2 / (k2 * 1e-6) / 8e12}
+        forms["mixed_over_fp64"] = forms["mixed"]["column_levels_per_s"] / forms["fp64"]["column_levels_per_s"]
+        errs = mixed_error(ctx, w2, dt2)
+        forms["mixed_error_vs_fp64"] = {
+            "worst_field": max(errs, key=lambda k: errs[k]["max_rel_to_field_max"]),
+            "max_rel_to_field_max": max(e["max_rel_to_field_max"] for e in errs.values()),
+            "per_field": errs}
+        line["forms"] = {f"{ncol2}x{nlev2}_dt{dt2:g}": forms}
+        del w2
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline()
+    ctx.close()
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -18,6 +18,14 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "libp3oracle.so")
 ORACLE_FMATH_SO = os.path.join(HERE, "libp3oracle_fmath.so")
+ORACLE_PMATH_SO = os.path.join(HERE, "libp3oracle_pmath.so")
+# Elementary-function flavours of the restatement:
+#   "glibc"    the C library, as the reference C++ on the CPU (the default);
+#   "fortran"  cbrt/expm1 spelled as the non-BFB Fortran reference does, for
+#              the bit-for-bit comparison with micro_p3.F90;
+#   "portable" scream_amd/csrc/p3_math.h, the functions the HIP FP64 form
+#              uses, for the bit-for-bit comparison with the GPU.
+FLAVOURS = {"glibc": ORACLE_SO, "fortran": ORACLE_FMATH_SO, "portable": ORACLE_PMATH_SO}
 REF_SO = os.path.join(HERE, "_ref", "libp3ref.so")
 
 _dp = C.POINTER(C.c_double)
@@ -74,10 +82,9 @@ _oracle = {}
 _ref = None
 
 
-def load_oracle(fortran_math=False):
-    """The restatement; ``fortran_math`` selects the build that spells cbrt and
-    expm1 the way the non-BFB Fortran reference does."""
-    so = ORACLE_FMATH_SO if fortran_math else ORACLE_SO
+def load_oracle(math="glibc"):
+    """The restatement built with the elementary functions of flavour ``math``."""
+    so = FLAVOURS[math]
     if so not in _oracle:
         if not os.path.exists(so):
             raise RuntimeError(f"{so} is not built (run `make -C oracle`)")
@@ -86,6 +93,8 @@ def load_oracle(fortran_math=False):
         lib.p3o_main.argtypes = [C.POINTER(_Runtime), C.POINTER(_Tables), C.POINTER(_Fields),
                                  C.POINTER(_Debug), C.c_int, C.c_int, C.c_double, C.c_int,
                                  C.c_int, C.c_int]
+        lib.p3o_elementary.restype = C.c_double
+        lib.p3o_elementary.argtypes = [C.c_int, C.c_double, C.c_double]
         lib.p3o_qv_sat_dry.restype = C.c_double
         lib.p3o_qv_sat_dry.argtypes = [C.c_double, C.c_double, C.c_int]
         lib.p3o_lookup_ice.argtypes = [C.c_double] * 4 + [_ip] * 3 + [_dp] * 3
@@ -125,9 +134,9 @@ def _tables(tab):
 
 
 def run_oracle(inputs: dict, tables: dict, dt: float, predict_nc=True, prescribed_ccn=False,
-               runtime=None, nthreads=0, fortran_math=False):
+               runtime=None, nthreads=0, math="glibc"):
     """p3_main on copies of ``inputs``; returns (fields dict, debug dict, n_errors)."""
-    lib = load_oracle(fortran_math)
+    lib = load_oracle(math)
     nlev, ncol = inputs["qc"].shape
     f = {k: np.array(inputs[k], dtype=np.float64, order="C") for k in STATE + INPUTS}
     for k in OUTS + HIST:

@@ -84,7 +84,21 @@ static const double lookup_table_1a_dum1_c = 4.135985029041767e+00;
  * (components/eam/src/physics/cam/bfb_math.inc:20-27).  Building with
  * -DP3O_FORTRAN_MATH selects the Fortran spelling so the restatement can be
  * compared with the Fortran reference bit for bit. */
-#ifdef P3O_FORTRAN_MATH
+#if defined(P3O_PORTABLE_MATH)
+/* Third flavour: the elementary functions of scream_amd/csrc/p3_math.h, which
+ * round identically on the CPU and on the GPU.  With them the restatement
+ * checks the HIP FP64 form bit for bit; the default flavour (glibc, as the
+ * reference) is the one pinned to the Fortran reference. */
+#include "../scream_amd/csrc/p3_math.h"
+#define exp(x) p3m_exp(x)
+#define log(x) p3m_log(x)
+#define log10(x) p3m_log10(x)
+#define pow(x, y) p3m_pow(x, y)
+#define tgamma(x) p3m_tgamma(x)
+#define tanh(x) p3m_tanh(x)
+#define P3_CBRT(x) p3m_cbrt(x)
+#define P3_EXPM1(x) p3m_expm1(x)
+#elif defined(P3O_FORTRAN_MATH)
 #define P3_CBRT(x) pow((x), 1.0 / 3.0)
 #define P3_EXPM1(x) (exp(x) - 1)
 #else
@@ -126,6 +140,22 @@ static double qv_sat_dry_e(double t, double p, int ice, int* err) {
 double p3o_qv_sat_dry(double t, double p, int ice) {
   int err = 0;
   return qv_sat_dry_e(t, p, ice, &err);
+}
+
+/* The elementary functions of this build flavour (for the math tests):
+ * 0 exp, 1 log, 2 log10, 3 pow, 4 cbrt, 5 tgamma, 6 expm1, 7 tanh. */
+double p3o_elementary(int fn, double x, double y) {
+  switch (fn) {
+    case 0: return exp(x);
+    case 1: return log(x);
+    case 2: return log10(x);
+    case 3: return pow(x, y);
+    case 4: return P3_CBRT(x);
+    case 5: return tgamma(x);
+    case 6: return P3_EXPM1(x);
+    case 7: return tanh(x);
+    default: return NAN;
+  }
 }
 
 /* ======================== tables ======================== */

@@ -103,6 +103,7 @@ int p3o_main(const p3o_runtime* rt, const p3o_tables* tab, const p3o_fields* f,
 
 /* Building blocks exported for unit tests. */
 double p3o_qv_sat_dry(double t, double p, int ice);
+double p3o_elementary(int fn, double x, double y);
 void p3o_lookup_ice(double qi, double ni, double qm, double rhop,
                     int* dumi, int* dumii, int* dumjj,
                     double* dum1, double* dum4, double* dum5);

@@ -275,15 +275,21 @@ __global__ __launch_bounds__(NT) void p3_main_kernel(const KParams p) {
       if (nsub >= kMaxSubsteps && dt_left > dt_left_tol) { err |= ERR_SUBSTEP_LIMIT; dt_left = 0; }
       const bool run = present && dt_left > dt_left_tol;
       if (!__syncthreads_or(run)) break;
-      // ---- velocities and CFL number on the thread's part of [k_top, k_bot]
+      // ---- velocities and CFL number on the thread's part of [k_top, k_bot].
+      // The upwind step below also reads the level under the window, whose
+      // velocity the reference zeroes with the whole column (P3I/p3_ice_sed_impl.hpp:120-125);
+      // it is written as 0 here so no velocity of an earlier species survives there.
       double co = 0;
       if (run) {
-        const int ka = max(k0, k_top), kb = min(k1 - 1, k_bot);
+        const int k_under = (k_bot == kbot) ? k_bot : k_bot + 1;
+        const int ka = max(k0, k_top), kb = min(k1 - 1, k_under);
         for (int k = ka; k <= kb; ++k) {
           const size_t x = at(k);
           const double rho = p.dpres[x] / p.dz[x] / gravit;
           double V1 = 0, V2 = 0;
-          if (sp == 0) {
+          if (k > k_bot) {
+            // below the window: no velocity
+          } else if (sp == 0) {
             // cloud_sedimentation, P3I/p3_cloud_sed_impl.hpp:85-112
             const double cfl = p.cld_frac_l[x];
             const double qc = p.qc[x], nc = p.nc[x];

@@ -37,15 +37,15 @@ __device__ bool part2_level(const Runtime& rt, const Tables& tb, double dt, bool
   R qc_incld = R(in0.qc), qr_incld = R(in0.qr), qi_incld = R(in0.qi), qm_incld = R(in0.qm);
   R nc_incld = R(in0.nc), nr_incld = R(in0.nr), ni_incld = R(in0.ni), bm_incld = R(in0.bm);
 
-  R qc2qr_accret_tend = 0, qr2qv_evap_tend = 0, qc2qr_autoconv_tend = 0, nc_accret_tend = 0,
-    nc_selfcollect_tend = 0, nc2nr_autoconv_tend = 0, nr_selfcollect_tend = 0, nr_evap_tend = 0,
-    ncautr = 0;
+  R qc2qr_accret_tend = 0, qc2qr_autoconv_tend = 0, nc_accret_tend = 0,
+    nc_selfcollect_tend = 0, nc2nr_autoconv_tend = 0, nr_selfcollect_tend = 0, ncautr = 0;
   R qi2qv_sublim_tend = 0, nr_ice_shed_tend = 0, qc2qi_hetero_freeze_tend = 0, qr2qi_collect_tend = 0,
     qc2qr_ice_shed_tend = 0, qi2qr_melt_tend = 0, qc2qi_collect_tend = 0, qr2qi_immers_freeze_tend = 0,
     qv2qi_nucleat_tend = 0, ni2nr_melt_tend = 0, nc_collect_tend = 0, ncshdc = 0,
     nc2ni_immers_freeze_tend = 0, nr_collect_tend = 0, ni_selfcollect_tend = 0, ni_nucleat_tend = 0,
     qv2qi_vapdep_tend = 0, qc2qi_berg_tend = 0, nr2ni_immers_freeze_tend = 0, ni_sublim_tend = 0;
   double rho_qm_cloud = 400;
+  double evap_d = 0, nr_evap_d = 0;  // rain evaporation rates (finished in FP64)
   bool wetgrowth = false;
   R mu_c = 0, lamc = 0, cdist = 0, cdist1 = 0, mu_r = 0, lamr = 0, cdistr = 0, logn0r = 0;
 
@@ -262,20 +262,28 @@ __device__ bool part2_level(const Runtime& rt, const Tables& tb, double dt, bool
         }
         if (eps_eff < R(1e-20)) eps_eff = R(1e-20);
         const R tau_eff = R(1) / eps_eff;
+        // The limits below can remove all rain; they run in FP64 on the FP64
+        // in-cloud mass (qr_incld is unchanged since entry) so that complete
+        // evaporation leaves the same clipped residual in both forms.
+        const double qr_incld_d = in0.qr;
+        const double inv_dt_d = 1 / dt;
+        double ev;
         if (qr_incld < R(1e-12) && s.qv / s.qv_sat_l < 0.999) {
-          qr2qv_evap_tend = qr_incld * inv_dt_r;
+          ev = qr_incld_d * inv_dt_d;
         } else {
           const R dt_over_tau = R(dt) / tau_eff;
           const R tscale_weight = -m_expm1(-dt_over_tau) / dt_over_tau;
           const R equilib_tend = -A_c / ab * tau_eff / tau_r;
           const R instant_tend = -ssat_r / (ab * tau_r);
-          qr2qv_evap_tend = instant_tend * tscale_weight + equilib_tend * (R(1) - tscale_weight);
+          ev = double(instant_tend * tscale_weight + equilib_tend * (R(1) - tscale_weight));
         }
-        if (qr2qv_evap_tend > -ssat_r * inv_dt_r / ab) qr2qv_evap_tend = -ssat_r * inv_dt_r / ab;
-        if (qr2qv_evap_tend < R(0)) qr2qv_evap_tend = R(0);
-        if (qr2qv_evap_tend > qr_incld * inv_dt_r) qr2qv_evap_tend = qr_incld * inv_dt_r;
-        qr2qv_evap_tend = qr2qv_evap_tend * (cfr - cld_frac) / cfr;
-        nr_evap_tend = qr2qv_evap_tend * (nr_incld / qr_incld);
+        const double abd = ab, cfd = cld_frac;
+        if (ev > -ssat_r_d * inv_dt_d / abd) ev = -ssat_r_d * inv_dt_d / abd;
+        if (ev < 0) ev = 0;
+        if (ev > qr_incld_d * inv_dt_d) ev = qr_incld_d * inv_dt_d;
+        ev = ev * (cld_frac_r - cfd) / cld_frac_r;
+        evap_d = ev;
+        nr_evap_d = ev * (double(nr_incld) / qr_incld_d);
       }
     }
 
@@ -321,7 +329,7 @@ __device__ bool part2_level(const Runtime& rt, const Tables& tb, double dt, bool
   // cloud_water_autoconversion, P3I/p3_autoconversion_impl.hpp:10-57
   {
     const double radius = rt.autoconversion_radius;
-    const R cons3 = R((radius == 25.0e-6) ? CONS3 : 1.0 / (CONS2 * ::pow(radius, 3.0)));
+    const R cons3 = R((radius == 25.0e-6) ? CONS3 : 1.0 / (CONS2 * p3m_pow(radius, 3.0)));
     if (qc_incld >= R(1e-8)) {
       qc2qr_autoconv_tend = R(1.0) * R(rt.autoconversion_prefactor) * m_pow(qc_incld, R(rt.autoconversion_qc_exponent)) *
                             m_pow(nc_incld * R(1.e-6) * R(rho), R(-rt.autoconversion_nc_exponent));
@@ -354,10 +362,10 @@ __device__ bool part2_level(const Runtime& rt, const Tables& tb, double dt, bool
   }
 
   // ---------------- FP64 from here: cell averages, limiters, updates ----------------
-  double t_qc2qr_accret = qc2qr_accret_tend, t_qr2qv_evap = qr2qv_evap_tend,
+  double t_qc2qr_accret = qc2qr_accret_tend, t_qr2qv_evap = evap_d,
          t_qc2qr_autoconv = qc2qr_autoconv_tend, t_nc_accret = nc_accret_tend,
          t_nc_selfcollect = nc_selfcollect_tend, t_nc2nr_autoconv = nc2nr_autoconv_tend,
-         t_nr_selfcollect = nr_selfcollect_tend, t_nr_evap = nr_evap_tend, t_ncautr = ncautr,
+         t_nr_selfcollect = nr_selfcollect_tend, t_nr_evap = nr_evap_d, t_ncautr = ncautr,
          t_qi2qv_sublim = qi2qv_sublim_tend, t_nr_ice_shed = nr_ice_shed_tend,
          t_qc2qi_hetero = qc2qi_hetero_freeze_tend, t_qr2qi_collect = qr2qi_collect_tend,
          t_qc2qr_ice_shed = qc2qr_ice_shed_tend, t_qi2qr_melt = qi2qr_melt_tend,

@@ -17,29 +17,31 @@
 #include <hip/hip_runtime.h>
 
 #include "p3_constants.hpp"
+#include "p3_math.h"
 
 namespace p3d {
 
 using namespace p3c;
 
 // ---------------------------------------------------------------- math shims
-// One spelling per precision so that the FP64 form calls exactly the double
-// routine and the mixed form exactly the float one.
-__device__ __forceinline__ double m_exp(double x) { return ::exp(x); }
+// FP64: the portable functions of p3_math.h, which round identically on the
+// CPU (the oracle's portable build) and here, so the FP64 form reproduces the
+// oracle bit for bit.  FP32 (mixed form): the hardware-backed float routines.
+__device__ __forceinline__ double m_exp(double x) { return p3m_exp(x); }
 __device__ __forceinline__ float m_exp(float x) { return ::expf(x); }
-__device__ __forceinline__ double m_log(double x) { return ::log(x); }
+__device__ __forceinline__ double m_log(double x) { return p3m_log(x); }
 __device__ __forceinline__ float m_log(float x) { return ::logf(x); }
-__device__ __forceinline__ double m_log10(double x) { return ::log10(x); }
+__device__ __forceinline__ double m_log10(double x) { return p3m_log10(x); }
 __device__ __forceinline__ float m_log10(float x) { return ::log10f(x); }
-__device__ __forceinline__ double m_pow(double x, double y) { return ::pow(x, y); }
+__device__ __forceinline__ double m_pow(double x, double y) { return p3m_pow(x, y); }
 __device__ __forceinline__ float m_pow(float x, float y) { return ::powf(x, y); }
-__device__ __forceinline__ double m_cbrt(double x) { return ::cbrt(x); }
+__device__ __forceinline__ double m_cbrt(double x) { return p3m_cbrt(x); }
 __device__ __forceinline__ float m_cbrt(float x) { return ::cbrtf(x); }
-__device__ __forceinline__ double m_sqrt(double x) { return ::sqrt(x); }
+__device__ __forceinline__ double m_sqrt(double x) { return ::sqrt(x); }  // IEEE correctly rounded
 __device__ __forceinline__ float m_sqrt(float x) { return ::sqrtf(x); }
-__device__ __forceinline__ double m_tgamma(double x) { return ::tgamma(x); }
+__device__ __forceinline__ double m_tgamma(double x) { return p3m_tgamma(x); }
 __device__ __forceinline__ float m_tgamma(float x) { return ::tgammaf(x); }
-__device__ __forceinline__ double m_expm1(double x) { return ::expm1(x); }
+__device__ __forceinline__ double m_expm1(double x) { return p3m_expm1(x); }
 __device__ __forceinline__ float m_expm1(float x) { return ::expm1f(x); }
 __device__ __forceinline__ double m_fabs(double x) { return ::fabs(x); }
 __device__ __forceinline__ float m_fabs(float x) { return ::fabsf(x); }
@@ -56,11 +58,11 @@ enum : unsigned { ERR_BAD_TEMPERATURE = 1u, ERR_SUBSTEP_LIMIT = 2u };
 __device__ __forceinline__ double mk_svp(double t, bool ice, unsigned& err) {
   if (t <= 0 || isnan(t)) err |= ERR_BAD_TEMPERATURE;
   if (ice && t < Tmelt) {
-    return ::exp(9.550426 - (5723.265 / t) + (3.53068 * ::log(t)) - (0.00728332 * t));
+    return p3m_exp(9.550426 - (5723.265 / t) + (3.53068 * p3m_log(t)) - (0.00728332 * t));
   }
-  const double logt = ::log(t);
-  return ::exp(54.842763 - (6763.22 / t) - (4.210 * logt) + (0.000367 * t) +
-               (::tanh(0.0415 * (t - 218.8)) *
+  const double logt = p3m_log(t);
+  return p3m_exp(54.842763 - (6763.22 / t) - (4.210 * logt) + (0.000367 * t) +
+               (p3m_tanh(0.0415 * (t - 218.8)) *
                 (53.878 - (1331.22 / t) - (9.44523 * logt) + 0.014025 * t)));
 }
 // qv_sat_dry, PSH:127-157

@@ -118,15 +118,16 @@ extern "C" int p3x_init(const p3x_table_data* t, p3x_context** out) {
 extern "C" void p3x_finalize(p3x_context* c) {
   if (!c) return;
   (void)hipDeviceSynchronize();
-  hipFree(c->d_ice);
-  hipFree(c->d_coll);
-  hipFree(c->d_vn);
-  hipFree(c->d_vm);
-  hipFree(c->d_revap);
-  hipFree(c->d_scratch);
-  hipFree(c->d_err);
-  if (c->ev0) hipEventDestroy(c->ev0);
-  if (c->ev1) hipEventDestroy(c->ev1);
+  // teardown: nothing useful can be done with a failure here
+  (void)hipFree(c->d_ice);
+  (void)hipFree(c->d_coll);
+  (void)hipFree(c->d_vn);
+  (void)hipFree(c->d_vm);
+  (void)hipFree(c->d_revap);
+  (void)hipFree(c->d_scratch);
+  (void)hipFree(c->d_err);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
   delete c;
 }
 

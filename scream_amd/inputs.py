@@ -74,8 +74,21 @@ def base_temperature(nlev: int):
     return t_mid, z_mid
 
 
-def make_columns(ncol: int, nlev: int, seed: int = 0, sigma: float = 1.0) -> dict:
-    """All p3_main input arrays for ``ncol`` columns, each shaped (nlev, ncol)."""
+def murphy_koop_ice(t: np.ndarray) -> np.ndarray:
+    """Saturation vapour pressure over ice [Pa], Murphy & Koop (2005) eq. 7."""
+    return np.exp(9.550426 - (5723.265 / t) + (3.53068 * np.log(t)) - (0.00728332 * t))
+
+
+def make_columns(ncol: int, nlev: int, seed: int = 0, sigma: float = 1.0,
+                 dry_every: int = 0, evap_every: int = 0) -> dict:
+    """All p3_main input arrays for ``ncol`` columns, each shaped (nlev, ncol).
+
+    ``dry_every = n`` makes every n-th column (0, n, 2n, ...) free of
+    hydrometeors at half of ice/liquid saturation, so p3_main skips it after
+    part 1.  ``evap_every = m`` makes every m-th column (offset 1) the same but
+    with a trace of rain (2e-14 kg/kg) at its lowest level, which evaporates
+    completely in part 2, so p3_main skips the column after part 2.
+    """
     rng = np.random.default_rng(seed)
     shape = (nlev, ncol)
     p_int, p_mid = pressure_levels(nlev)
@@ -98,6 +111,23 @@ def make_columns(ncol: int, nlev: int, seed: int = 0, sigma: float = 1.0) -> dic
     ni = np.where(qi > 0, 1e5, 0.0)
     qm = 0.3 * qi
     bm = qm / 500.0
+
+    special = np.zeros(ncol, bool)
+    if dry_every:
+        special[::dry_every] = True
+    evap = np.zeros(ncol, bool)
+    if evap_every:
+        evap[1::evap_every] = True
+        special |= evap
+    if special.any():
+        es = np.where(T < TMELT, murphy_koop_ice(T), murphy_koop_liquid(T))
+        qv_dry = 0.5 * EP_2 * es / np.maximum(pres, 1e-3)
+        for a in (qc, nc, qr, nr, qi, ni, qm, bm):
+            a[:, special] = 0.0
+        qv[:, special] = qv_dry[:, special]
+        rh[:, special] = 0.5
+        qr[nlev - 1, evap] = 2e-14
+        nr[nlev - 1, evap] = 1e3
 
     cf = np.clip((rh - 0.8) / 0.2, 0.0, 1.0)
     cld_l = np.maximum(cf, 1e-4)

@@ -96,6 +96,13 @@ def load() -> C.CDLL:
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m scream_amd.build`")
+        # torch ships its own libamdhip64.so.7.  Importing it first makes that the
+        # process's one HIP runtime, which libp3x.so then binds to by soname; the
+        # other order would load a second HIP/HSA runtime next to torch's.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         lib = C.CDLL(LIB_PATH)
         lib.p3x_runtime_defaults.argtypes = [C.POINTER(Runtime)]
         lib.p3x_launch_defaults.argtypes = [C.POINTER(Launch)]

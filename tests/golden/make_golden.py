@@ -1,0 +1,110 @@
+"""Writes the golden fixtures of tests/golden/ from the reference's own Fortran P3.
+
+Run in a container that has /root/reference (the recipe compiles
+components/eam/src/physics/p3/scream/micro_p3.F90 and its modules where they
+lie; see oracle/Makefile):
+
+    make -C oracle ref && python tests/golden/make_golden.py
+
+Fixtures (all data, no code):
+  p3_main_<case>.npz   inputs and every output of Fortran p3_main for one
+                       seeded column set (inputs from scream_amd.inputs)
+  tables.npz           the rain tables the Fortran p3_init builds, and SHA-256
+                       digests of its ice / collection tables
+  lookup.npz           Fortran find_lookupTable_indices_1a/1b +
+                       access_lookup_table(_coll) on the literal inputs of
+                       components/eamxx/src/physics/p3/tests/p3_ice_tables_unit_tests.cpp:111-155
+"""
+import ctypes as C
+import hashlib
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+import oracle  # noqa: E402
+from scream_amd.inputs import make_columns  # noqa: E402
+
+TABLE_DIR = os.path.join(ROOT, "scream_amd", "data")
+
+# name: (ncol, nlev, dt, predictNc, prescribedCCN, seed)
+CASES = {
+    "l72_dt300_pnc": (24, 72, 300.0, True, False, 101),
+    "l72_dt1800_nopnc": (24, 72, 1800.0, False, False, 102),
+    "l128_dt100_pnc_ccn": (24, 128, 100.0, True, True, 103),
+    "l72_dt1800_nopnc_ccn": (24, 72, 1800.0, False, True, 104),
+    "l128_dt1800_pnc": (24, 128, 1800.0, True, False, 105),
+}
+
+# components/eamxx/src/physics/p3/tests/p3_ice_tables_unit_tests.cpp:111-155
+LOOKUP_ICE = [
+    (0.971E-07, 0.657E+06, 0.971E-07, 0.900E+03), (0.510E-02, 0.454E+06, 0.714E-05, 0.500E+02),
+    (0.500E-07, 0.545E+06, 0.000E+00, 0.000E+00), (0.136E-08, 0.487E+07, 0.811E-10, 0.500E+02),
+    (0.971E-07, 0.657E+06, 0.971E-07, 0.900E+03), (0.510E-02, 0.454E+01, 0.714E-05, 0.500E+02),
+    (0.500E-07, 0.545E+06, 0.000E+00, 0.000E+00), (0.136E-08, 0.487E+06, 0.811E-10, 0.500E+02),
+    (0.971E-07, 0.657E+06, 0.271E-06, 0.900E+03), (0.510E-02, 0.454E+06, 0.714E-05, 0.500E+02),
+    (0.500E-07, 0.545E+06, 0.000E+00, 0.000E+00), (0.136E-08, 0.487E+06, 0.811E-10, 0.500E+02),
+    (0.971E-07, 0.657E+06, 0.971E-07, 0.200E+04), (0.510E-02, 0.454E+06, 0.714E-05, 0.500E+02),
+    (0.500E-07, 0.545E+06, 0.000E+00, 0.000E+00), (0.136E-08, 0.487E+06, 0.811E-10, 0.500E+02),
+]
+LOOKUP_RAIN = [
+    (0.263E-05, 0.100E+07), (0.100E-01, 0.100E+07), (0.000E+00, 0.100E-15), (0.263E-05, 0.100E+07),
+    (0.263E-05, 0.100E+07), (0.100E-01, 0.100E+07), (0.000E+00, 0.0), (0.263E-05, 0.100E+07),
+    (0.263E-05, 0.100E+07), (0.100E-01, 0.100E+07), (0.000E+00, 0.100E-15), (0.263E-05, 0.100E+07),
+    (0.263E-05, 0.100E+07), (0.100E-01, 0.100E+07), (0.000E+00, 0.100E-15), (0.263E-05, 0.100E+07),
+]
+
+OUT_FIELDS = oracle.STATE + oracle.OUTS + oracle.HIST
+
+
+def main():
+    lib = oracle.load_ref(TABLE_DIR)
+    for name, (ncol, nlev, dt, pnc, pccn, seed) in CASES.items():
+        inp = make_columns(ncol, nlev, seed=seed, dry_every=6, evap_every=7)
+        out = oracle.run_ref(inp, TABLE_DIR, dt, pnc, pccn)
+        data = {f"in_{k}": inp[k] for k in oracle.STATE + oracle.INPUTS}
+        data.update({f"out_{k}": out[k] for k in OUT_FIELDS})
+        data["meta"] = np.array([ncol, nlev, dt, int(pnc), int(pccn), seed], dtype=np.float64)
+        np.savez_compressed(os.path.join(HERE, f"p3_main_{name}.npz"), **data)
+        print("wrote", name)
+
+    t = oracle.ref_tables(TABLE_DIR)
+    np.savez_compressed(
+        os.path.join(HERE, "tables.npz"), vn=t["vn"], vm=t["vm"], revap=t["revap"], mu_r=t["mu_r"],
+        ice_sha256=np.frombuffer(hashlib.sha256(np.ascontiguousarray(t["ice"]).tobytes()).digest(), np.uint8),
+        coll_sha256=np.frombuffer(hashlib.sha256(np.ascontiguousarray(t["coll"]).tobytes()).digest(), np.uint8))
+    print("wrote tables")
+
+    n = len(LOOKUP_ICE)
+    ice_idx = np.zeros((n, 3), np.int32)
+    ice_w = np.zeros((n, 3))
+    ice_proc = np.zeros((n, 12))
+    coll_j = np.zeros(n, np.int32)
+    coll_d3 = np.zeros(n)
+    coll_proc = np.zeros((n, 2))
+    for s, (qi, ni, qm, rhop) in enumerate(LOOKUP_ICE):
+        qr, nr = LOOKUP_RAIN[s]
+        for idx in range(12):
+            a, b, cc = C.c_int(), C.c_int(), C.c_int()
+            d1, d4, d5, p = C.c_double(), C.c_double(), C.c_double(), C.c_double()
+            lib.p3ref_lookup_ice(qi, ni, qm, rhop, idx, C.byref(a), C.byref(b), C.byref(cc),
+                                 C.byref(d1), C.byref(d4), C.byref(d5), C.byref(p))
+            ice_idx[s] = (a.value, b.value, cc.value)
+            ice_w[s] = (d1.value, d4.value, d5.value)
+            ice_proc[s, idx] = p.value
+        for idx in range(2):
+            j, d3, p = C.c_int(), C.c_double(), C.c_double()
+            lib.p3ref_lookup_coll(qi, ni, qm, rhop, qr, nr, idx, C.byref(j), C.byref(d3), C.byref(p))
+            coll_j[s], coll_d3[s], coll_proc[s, idx] = j.value, d3.value, p.value
+    np.savez_compressed(os.path.join(HERE, "lookup.npz"), ice_in=np.array(LOOKUP_ICE),
+                        rain_in=np.array(LOOKUP_RAIN), ice_idx=ice_idx, ice_w=ice_w,
+                        ice_proc=ice_proc, coll_j=coll_j, coll_d3=coll_d3, coll_proc=coll_proc)
+    print("wrote lookup")
+
+
+if __name__ == "__main__":
+    main()
