@@ -36,6 +36,12 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-forms", action="store_true", help="skip the 16384x128 forms")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain 1 GiB device copy (HBM counter calibration)")
+    ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="turn two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) of "
+                         "'bench.py --no-cpu --no-forms' and of 'bench.py --copy-probe' "
+                         "into profiles/traffic.json")
     return ap.parse_args()
 
 
@@ -137,16 +143,83 @@ def cpu_baseline():
                       f"dt 1800 s, median of 5 runs ({med:.3f} s each)"}
 
 
+COPY_PROBE_BYTES = 1 << 30
+
+
+def copy_probe():
+    """A plain device-to-device copy of 1 GiB, three times: under rocprofv3 --pmc its
+    FETCH_SIZE / WRITE_SIZE per dispatch calibrate the counters (1 GiB read, 1 GiB written)."""
+    import torch
+    n = COPY_PROBE_BYTES // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": COPY_PROBE_BYTES}))
+
+
+def _pmc_means(directory, counter):
+    """Mean counter value per dispatch, by kernel name, from rocprofv3 csv output."""
+    import csv
+    import glob
+    sums = {}
+    for path in glob.glob(os.path.join(directory, "**", "*counter_collection.csv"), recursive=True):
+        with open(path) as fh:
+            for row in csv.DictReader(fh):
+                if row.get("Counter_Name") != counter:
+                    continue
+                k = row["Kernel_Name"]
+                s = sums.setdefault(k, [0.0, 0])
+                s[0] += float(row["Counter_Value"])
+                s[1] += 1
+    return {k: v[0] / v[1] for k, v in sums.items()}
+
+
+def summarize_pmc(fetch_dir, write_dir):
+    """HBM bytes per step of the headline workload from FETCH_SIZE and WRITE_SIZE (KiB per
+    dispatch), each scaled by the bytes-per-KiB the same counter reports for the copy probe."""
+    fetch, write = _pmc_means(fetch_dir, "FETCH_SIZE"), _pmc_means(write_dir, "WRITE_SIZE")
+    pick = lambda d, key: {k: v for k, v in d.items() if key in k}
+    copy_f = max(pick(fetch, "copy").values() or pick(fetch, "elementwise").values())
+    copy_w = max(pick(write, "copy").values() or pick(write, "elementwise").values())
+    scale_f = COPY_PROBE_BYTES / copy_f  # bytes per counter KiB, reads
+    scale_w = COPY_PROBE_BYTES / copy_w  # bytes per counter KiB, writes
+    kern = {}
+    for name in ("p3_part12_kernel", "p3_sed_kernel"):
+        f = pick(fetch, name)
+        w = pick(write, name)
+        if f and w:
+            kern[name] = {"FETCH_SIZE_KiB": max(f.values()), "WRITE_SIZE_KiB": max(w.values()),
+                          "bytes": max(f.values()) * scale_f + max(w.values()) * scale_w}
+    out = {"g64_fp64": {
+        "workload": "p3_main FP64 drop-in, 21600x72, dt=1800 s (bench.py --no-cpu --no-forms)",
+        "copy_probe": {"bytes_each_way": COPY_PROBE_BYTES, "FETCH_SIZE_KiB": copy_f,
+                       "WRITE_SIZE_KiB": copy_w, "read_bytes_per_KiB": scale_f,
+                       "write_bytes_per_KiB": scale_w},
+        "kernels": kern,
+        "bytes_per_step": sum(v["bytes"] for v in kern.values())}}
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    with open(path, "w") as fh:
+        json.dump(out, fh, indent=1)
+    print(json.dumps(out))
+
+
 def traffic_for(workload):
+    """HBM bytes per step measured by rocprofv3 counters (profiles/traffic.json), or None."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
-        return json.load(fh).get(workload)
+        rec = json.load(fh).get(workload)
+    return None if rec is None else rec.get("bytes_per_step")
 
 
 def main():
     args = parse()
+    if args.summarize_pmc:
+        summarize_pmc(*args.summarize_pmc)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "P3 columns are independent, so N GPUs would each run this workload on their "
@@ -155,6 +228,9 @@ def main():
     import torch
     from scream_amd import p3x
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    if args.copy_probe:
+        copy_probe()
+        return
     ctx = p3x.Context()
 
     ncol, nlev, dt = 21600, 72, 1800.0

@@ -137,8 +137,13 @@ enum { P3X_FP64 = 0, P3X_MIXED = 1 };
 typedef struct p3x_launch {
   int precision;           /* P3X_FP64 (drop-in) or P3X_MIXED (FP32 bulk rates)  */
   int ld;                  /* row stride of the level-major arrays; 0 means nj    */
-  int columns_per_block;   /* 0 = tuned default; else 16, 32 or 64                */
-  int threads_per_block;   /* 0 = tuned default; else 128, 256 or 512             */
+  /* part 1 + part 2 kernel: columns per block x threads per block, both 0 for
+   * the default, else one of 16x128, 16x256, 32x256, 64x256, 32x512, 64x512 */
+  int columns_per_block;
+  int threads_per_block;
+  /* sedimentation + part 3 kernel: lanes per column, 0 for the default, else
+   * 16, 32 or 64; nk may not exceed 8 x this (nk <= 512 in any case)         */
+  int sed_lanes_per_column;
   void* stream;            /* hipStream_t; NULL = default stream                  */
   int async;               /* 1: return after enqueueing (status from p3x_sync)   */
   /* optional device arrays for the integer diagnostics (NULL = off):
@@ -180,6 +185,50 @@ int p3x_main(p3x_context* ctx, const p3x_runtime* runtime,
 
 /* Wait for the context's enqueued work and return its status. */
 int p3x_sync(p3x_context* ctx);
+
+/* ---- batched form (no reference counterpart: the reference runs one column
+ * set per p3_main call).  Many independent column sets -- each with its own
+ * nj, nk, dt and flags -- advance one P3 step each in one launch per kernel,
+ * optionally captured once into a HIP graph and replayed.  Every set's
+ * results equal those of a p3x_main call on that set alone (same kernels,
+ * same per-column arithmetic). ---- */
+typedef struct p3x_column_set {
+  p3x_prognostic_state prognostic_state;
+  p3x_diagnostic_inputs diagnostic_inputs;
+  p3x_diagnostic_outputs diagnostic_outputs;
+  p3x_history_only history_only;
+  p3x_infrastructure infrastructure;  /* dt, predictNc, prescribedCCN, exchange_zeroed_on_entry */
+  int nj, nk, ld;                     /* ld 0 means nj                                       */
+  int *col_status, *sed_substeps, *ice_table_idx;  /* optional, as in p3x_launch         */
+} p3x_column_set;
+
+typedef struct p3x_batch_options {
+  int precision;             /* P3X_FP64 or P3X_MIXED                                     */
+  int columns_per_block;     /* part 1 + part 2 shape, as in p3x_launch (0 = default)    */
+  int threads_per_block;
+  int sed_lanes_per_column;  /* 0 = default                                               */
+  int blocks_per_cu;         /* persistent grid of each launch: this many blocks per CU;  */
+                             /* 0 = as many as fit (occupancy); -1 = one block per unit   */
+  int use_graph;             /* 1 = capture the launches into a HIP graph once, replay it */
+} p3x_batch_options;
+
+typedef struct p3x_batch p3x_batch;
+
+void p3x_batch_defaults(p3x_batch_options* opts);
+/* The descriptors are copied; the arrays they point to must stay valid while
+ * the batch exists.  Allocates the batch's scratch and (use_graph) its graph. */
+int p3x_batch_create(p3x_context* ctx, const p3x_runtime* runtime, const p3x_column_set* sets,
+                     int nsets, const p3x_batch_options* opts, p3x_batch** batch);
+/* Enqueue one step of every set on stream (NULL = default); returns after
+ * enqueueing.  p3x_sync(ctx) waits and returns the status. */
+int p3x_batch_run(p3x_batch* batch, void* stream);
+/* Autotuner: times candidate shapes (part 1 + part 2 block shape, sedimentation
+ * lanes per column, persistent blocks per CU) on the batch's own data, keeps the
+ * fastest, and leaves every set's state as it was.  reps timed steps per
+ * candidate; chosen / best_us (optional) receive the result. */
+int p3x_batch_tune(p3x_batch* batch, void* stream, int reps, p3x_batch_options* chosen,
+                   double* best_us);
+void p3x_batch_destroy(p3x_batch* batch);
 
 const char* p3x_strerror(int status);
 

@@ -1,0 +1,52 @@
+// p3_batch.hpp -- persistent kernels that run one P3 step of many independent
+// column sets (each with its own ncol, nlev, dt and flags) in one launch each.
+//
+// A set's work is cut into the same blocks its single-set launch would use
+// ("virtual blocks").  first[s] is the index of set s's first virtual block in
+// the concatenation of all sets (first[nsets] = total).  A persistent grid of
+// gridDim.x blocks walks the virtual blocks round-robin; a block finds the set
+// of its virtual block by binary search over first[].
+//
+// The set descriptors are read through the constant address space: the
+// compiler then loads them with scalar loads and knows that the field pointers
+// inside them address global memory (global rather than flat loads).
+#pragma once
+
+#include "p3_sed.hpp"
+
+namespace p3d {
+
+__device__ __forceinline__ int find_set(const int* __restrict__ first, int nsets, int vb) {
+  int lo = 0, hi = nsets - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (first[mid] <= vb) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <class R, int CB, int NT>
+__global__ __launch_bounds__(NT) void p3_part12_batch(const KParams* __restrict__ sets,
+                                                      const int* __restrict__ first, int nsets) {
+  __shared__ int s_i[NT];
+  const auto sets4 = (const __attribute__((address_space(4))) KParams*)sets;
+  const int total = first[nsets];
+  for (int vb = blockIdx.x; vb < total; vb += gridDim.x) {
+    const int s = find_set(first, nsets, vb);
+    part12_block<R, CB, NT>(*(const KParams*)(sets4 + s), vb - first[s], s_i);
+  }
+}
+
+template <class R, int LPC, int L, int WPB>
+__global__ __launch_bounds__(64 * WPB) void p3_sed_batch(const KParams* __restrict__ sets,
+                                                         const int* __restrict__ first, int nsets) {
+  const auto sets4 = (const __attribute__((address_space(4))) KParams*)sets;
+  const int total = first[nsets];
+  for (int vb = blockIdx.x; vb < total; vb += gridDim.x) {
+    const int s = find_set(first, nsets, vb);
+    sed_block<R, LPC, L, WPB>(*(const KParams*)(sets4 + s), vb - first[s]);
+  }
+}
+
+}  // namespace p3d

@@ -4,5 +4,7 @@
 namespace p3d {
 #define REAL float
 #define LAUNCH_NAME launch_p3_mixed
+#define LAUNCH_BATCH_NAME launch_p3_batch_mixed
+#define OCC_NAME occupancy_p3_mixed
 #include "p3_kernel_variants.inc"
 }  // namespace p3d

@@ -3,13 +3,58 @@
 
 #include <hip/hip_runtime.h>
 
-#include "p3_kernel.hpp"
+#include "p3_batch.hpp"
 
 namespace p3d {
 
-// Block shapes compiled into the library: columns per block x threads per block.
-// Returns hipErrorInvalidValue for a shape that is not compiled in.
-hipError_t launch_p3_fp64(const KParams& p, int columns_per_block, int threads_per_block, hipStream_t s);
-hipError_t launch_p3_mixed(const KParams& p, int columns_per_block, int threads_per_block, hipStream_t s);
+// Shape of one step's two launches.
+//   part 1 + part 2: cb columns per block x nt threads per block
+//     (compiled: 16x128, 16x256, 32x256, 64x256, 32x512, 64x512);
+//   sedimentation + part 3: lpc lanes per column (16, 32 or 64), each lane
+//     holding L = ceil(nlev / lpc) levels (at most 8), 4 wavefronts per block.
+struct Shape {
+  int cb, nt, lpc;
+};
+
+// Largest levels-per-lane count of the sedimentation kernel.
+constexpr int kMaxLevelsPerLane = 8;
+// Wavefronts per block of the sedimentation kernel.
+constexpr int kSedWaves = 4;
+
+// Levels per lane the sedimentation kernel is instantiated with for nlev
+// levels on lpc lanes (0 if nlev does not fit).
+inline int sed_levels_per_lane(int nlev, int lpc) {
+  const int l = (nlev + lpc - 1) / lpc;
+  if (l <= 6) return l < 1 ? 1 : l;
+  return l <= kMaxLevelsPerLane ? kMaxLevelsPerLane : 0;
+}
+
+// hipErrorInvalidValue for a shape that is not compiled in.
+hipError_t launch_p3_fp64(const KParams& p, const Shape& sh, hipStream_t s);
+hipError_t launch_p3_mixed(const KParams& p, const Shape& sh, hipStream_t s);
+
+// ---- batched form (p3_batch.hpp)
+// Sets whose sedimentation kernel uses the same levels-per-lane count.
+struct SedClass {
+  int L;
+  const KParams* sets;  // device array
+  const int* first;     // device array, nsets + 1 entries
+  int nsets;
+  int grid;             // persistent blocks
+};
+struct BatchPlan {
+  Shape shape;
+  const KParams* sets;  // device array, all sets (part 1 + part 2 launch)
+  const int* first;     // device array, nsets + 1 entries
+  int nsets;
+  int grid;             // persistent blocks of the part 1 + part 2 launch
+  int nclass;
+  SedClass cls[kMaxLevelsPerLane];
+};
+hipError_t launch_p3_batch_fp64(const BatchPlan& b, hipStream_t s);
+hipError_t launch_p3_batch_mixed(const BatchPlan& b, hipStream_t s);
+// Resident blocks per CU of the batched kernels for this shape (occupancy API).
+hipError_t occupancy_p3_fp64(const Shape& sh, int L, int* blocks12, int* blocks_sed);
+hipError_t occupancy_p3_mixed(const Shape& sh, int L, int* blocks12, int* blocks_sed);
 
 }  // namespace p3d

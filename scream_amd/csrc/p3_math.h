@@ -129,29 +129,33 @@ P3M_FN double p3m_log10(double x) {
 }
 
 /* pow for the arguments P3 uses (x >= 0): exp(y log x) with log x in
- * double-double and the product's rounding error carried into exp. */
+ * double-double and the product's rounding error carried into exp.  A negative
+ * base with an integral exponent is |x|^y with the sign of (-1)^y.  (No
+ * recursion, so that the device compiler can inline it everywhere.) */
 P3M_FN double p3m_pow(double x, double y) {
   if (y == 0.0) return 1.0;
   if (!(x == x) || !(y == y)) return x + y;
   if (x == 1.0) return 1.0;
   if (y == 1.0) return x;
   if (y == 2.0) return x * x;
+  double sgn = 1.0;
   if (x < 0) {
     const double yi = (double)(int64_t)y;
     if (yi != y || y > 9007199254740992.0 || y < -9007199254740992.0) return p3m_nan();
-    const double r = p3m_pow(-x, y);
-    return ((int64_t)y & 1) ? -r : r;
+    sgn = ((int64_t)y & 1) ? -1.0 : 1.0;
+    x = -x;
+    if (x == 1.0) return sgn;
   }
   if (x == 0.0) return y > 0 ? 0.0 : p3m_inf();
-  if (p3m_bits(x) == 0x7ff0000000000000ULL) return y > 0 ? x : 0.0;
+  if (p3m_bits(x) == 0x7ff0000000000000ULL) return sgn * (y > 0 ? x : 0.0);
   double lh, ll;
   p3m_log_dd(x, &lh, &ll);
   const double z = y * lh;
   const double zl = fma(y, lh, -z) + y * ll;
-  if (z > 709.8) return p3m_inf();
-  if (z < -745.2) return 0.0;
+  if (z > 709.8) return sgn * p3m_inf();
+  if (z < -745.2) return sgn * 0.0;
   const double e = p3m_exp(z);
-  return fma(e, zl, e);
+  return sgn * fma(e, zl, e);
 }
 
 /* cbrt: inverse cube root y ~ a^(-1/3) from the exponent bits, refined by the

@@ -1,0 +1,440 @@
+// p3_sed.hpp -- second kernel of a P3 step: cloud, rain and ice sedimentation,
+// homogeneous freezing and p3_main_part3, with the surface precipitation.
+//
+// Work decomposition: one column per group of LPC lanes (LPC = 16, 32 or 64,
+// aligned inside the 64-lane wavefront); lane j of a group owns the L
+// contiguous levels j*L .. j*L+L-1 of its column and keeps their
+// sedimentation state in registers for the whole substep loop.  Everything a
+// substep needs from other levels goes through cross-lane operations inside
+// the group: the CFL maximum and find_top/find_bottom are butterfly
+// reductions, and the upwind flux entering a lane's first level is the last
+// level's flux of the lane above (one shuffle).  A column therefore runs
+// exactly its own number of substeps, without block barriers, and reads its
+// state from HBM once per species.
+//
+// Follows components/eamxx/src/physics/p3/impl/p3_main_impl.hpp:262-312
+// (sedimentation, homogeneous freezing, part 3), with
+// p3_cloud_sed_impl.hpp:14-147, p3_rain_sed_impl.hpp:14-170,
+// p3_ice_sed_impl.hpp:66-270, p3_upwind_impl.hpp:14-151 and
+// p3_main_impl_part3.hpp:18-204.
+#pragma once
+
+#include "p3_kernel.hpp"
+
+namespace p3d {
+
+template <int LPC>
+__device__ __forceinline__ int grp_min(int v) {
+#pragma unroll
+  for (int o = LPC / 2; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+  return v;
+}
+template <int LPC>
+__device__ __forceinline__ int grp_max(int v) {
+#pragma unroll
+  for (int o = LPC / 2; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+  return v;
+}
+template <int LPC>
+__device__ __forceinline__ double grp_maxd(double v) {
+#pragma unroll
+  for (int o = LPC / 2; o > 0; o >>= 1) {
+    const double w = __shfl_xor(v, o, 64);
+    v = w > v ? w : v;
+  }
+  return v;
+}
+
+// Per-lane geometry of the owned levels.
+template <int L>
+struct Geo {
+  double rho[L], inv_rho[L], inv_dz[L];
+};
+
+// One species' sedimentation over the group's column.  SP = 0 cloud, 1 rain,
+// 2 ice.  On return the species' fields hold the sedimented values (stored to
+// HBM), pflux accumulates the rain flux at the interface below each owned level,
+// and prt is the mass that left through the surface.
+template <int SP, class R, int LPC, int L>
+__device__ __forceinline__ void sediment(const KParams& p, int i, int kbase, const Geo<L>& g,
+                                         double (&pflux)[L], double& prt, int& nsub_out,
+                                         bool& present_out, unsigned& err) {
+  constexpr int NF = SP == 2 ? 4 : 2;  // fields held (cloud always holds nc: its DSD rewrites it)
+  const int nlev = p.nlev, kbot = nlev - 1;
+  const size_t ld = (size_t)p.ld;
+  const Runtime& rt = p.rt;
+  const int nfield = SP == 2 ? 4 : (SP == 1 ? 2 : (p.predictNc ? 2 : 1));
+  double* fld[NF];
+  if (SP == 0) { fld[0] = p.qc; fld[1] = p.nc; }
+  if (SP == 1) { fld[0] = p.qr; fld[1] = p.nr; }
+  if (SP == 2) { fld[0] = p.qi; fld[1] = p.ni; fld[2] = p.qm; fld[3] = p.bm; }
+  const double* cfp = SP == 0 ? p.cld_frac_l : (SP == 1 ? p.cld_frac_r : p.cld_frac_i);
+  const int coef_f = SP == 0 ? SC_ACN : (SP == 1 ? SC_RHOFACR : SC_RHOFACI);
+
+  double q[NF][L], cf[L], coef[L], cfl_ice[L];
+  int my_top = nlev, my_bot = -1;
+#pragma unroll
+  for (int t = 0; t < L; ++t) {
+    const int k = kbase + t;
+    const bool ok = k < nlev;
+    const size_t x = (size_t)(ok ? k : 0) * ld + i;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) q[f][t] = ok ? fld[f][x] : 0.0;
+    cf[t] = ok ? cfp[x] : 1.0;
+    coef[t] = ok ? p.scratch[((size_t)coef_f * nlev + (ok ? k : 0)) * ld + i] : 0.0;
+    cfl_ice[t] = (SP == 2 && ok) ? p.cld_frac_l[x] : 1.0;
+    // find_top / find_bottom, P3I/p3_find_impl.hpp:14-92
+    if (ok && q[0][t] >= QSMALL) {
+      if (k < my_top) my_top = k;
+      my_bot = k;
+    }
+  }
+  const int k_top = grp_min<LPC>(my_top);
+  int k_bot = grp_max<LPC>(my_bot);
+  const bool present = k_top < nlev;
+  present_out = present;
+  const int j = kbase / L;
+
+  double dt_left = p.dt, prt_accum = 0;
+  bool first = true;
+  int nsub = 0;
+#pragma unroll 1
+  while (present && dt_left > dt_left_tol) {
+    // The substep count is bounded by the CFL number (a few hundred at most for
+    // physical inputs); the cap only guarantees termination for corrupt inputs.
+    if (nsub >= kMaxSubsteps) { err |= ERR_SUBSTEP_LIMIT; break; }
+    double V1[L], V2[L];
+    double co = 0;
+#pragma unroll
+    for (int t = 0; t < L; ++t) {
+      const int k = kbase + t;
+      V1[t] = 0;
+      V2[t] = 0;
+      if (k >= k_top && k <= k_bot) {  // k_bot < nlev, so k is a real level
+        const double rho = g.rho[t];
+        if (SP == 0) {
+          // cloud_sedimentation, P3I/p3_cloud_sed_impl.hpp:85-112
+          const double cfl = cf[t];
+          const double qc = q[0][t], nc = q[1][t];
+          double qc_incld, nc_incld;
+          if (first) {
+            const double icfl = 1 / cfl;
+            if (qc >= QSMALL) { qc_incld = vmin(qc * icfl, incloud_limit); nc_incld = vmax(nc * icfl, 0.0); }
+            else { qc_incld = 0; nc_incld = 0; }
+          } else {
+            qc_incld = qc / cfl;
+            nc_incld = nc / cfl;
+          }
+          if (qc_incld > QSMALL) {
+            R ncr = R(nc_incld), mu_c, lamc, cdist, cdist1;
+            get_cloud_dsd2<R>(R(qc_incld), ncr, mu_c, R(rho), lamc, cdist, cdist1);
+            q[1][t] = double(ncr) * cfl;
+            const R dum = R(1) / m_pow(lamc, R(bcn));
+            const R acn = R(coef[t]);
+            V1[t] = double(acn * m_tgamma(R(4 + bcn) + mu_c) * dum / m_tgamma(mu_c + R(4)));
+            if (p.predictNc) V2[t] = double(acn * m_tgamma(R(1 + bcn) + mu_c) * dum / m_tgamma(mu_c + R(1)));
+          }
+        } else if (SP == 1) {
+          // rain_sedimentation + compute_rain_fall_velocity, P3I/p3_rain_sed_impl.hpp:14-34,104-126
+          const double cfr = cf[t];
+          const double qr = q[0][t], nr = q[1][t];
+          double qr_incld, nr_incld;
+          if (first) {
+            const double icfr = 1 / cfr;
+            if (qr >= QSMALL) { qr_incld = vmin(qr * icfr, precip_limit); nr_incld = vmax(nr * icfr, 0.0); }
+            else { qr_incld = 0; nr_incld = 0; }
+          } else {
+            qr_incld = qr / cfr;
+            nr_incld = nr / cfr;
+          }
+          if (qr_incld > QSMALL) {
+            R nrr = R(nr_incld), mu_r = 0, lamr;
+            get_rain_dsd2<R>(rt, R(qr_incld), nrr, mu_r, lamr);
+            const Tab3<R> t3 = lookup3<R>(mu_r, lamr);
+            const R rhofacr = R(coef[t]);
+            V1[t] = double(apply_table3<R>(p.tab.vm, t3) * rhofacr);
+            V2[t] = double(apply_table3<R>(p.tab.vn, t3) * rhofacr);
+            q[1][t] = double(nrr) * cfr;
+          }
+        } else {
+          // ice_sedimentation, P3I/p3_ice_sed_impl.hpp:134-166
+          const double cfi = cf[t];
+          const double qi = q[0][t], ni = q[1][t], qm = q[2][t], bm = q[3][t];
+          double qi_incld, ni_incld, qm_incld, bm_incld;
+          if (first) {
+            const double icfi = 1 / cfi, icfl = 1 / cfl_ice[t];
+            if (qi >= QSMALL) { qi_incld = vmin(qi * icfi, incloud_limit); ni_incld = vmax(ni * icfi, 0.0); }
+            else { qi_incld = 0; ni_incld = 0; }
+            if (qi >= QSMALL && qm >= QSMALL) { qm_incld = qm * icfi; bm_incld = vmin(vmax(bm * icfl, 0.0), incloud_limit); }
+            else { qm_incld = 0; bm_incld = 0; }
+          } else {
+            qi_incld = qi / cfi;
+            ni_incld = ni / cfi;
+            qm_incld = qm / cfi;
+            bm_incld = bm / cfi;
+          }
+          if (qi_incld > QSMALL) {
+            R nir = vmax(R(ni_incld), R(NSMALL));
+            R qmr = R(qm_incld), bmr = R(bm_incld);
+            const R qir = R(qi_incld);
+            const R rhop = calc_bulk_rho_rime<R>(rt, qir, qmr, bmr);
+            q[2][t] = double(qmr) * cfi;
+            q[3][t] = double(bmr) * cfi;
+            const TabIce<R> ti = lookup_ice<R>(qir, nir, qmr, rhop);
+            const R t_ni_fall = apply_table_ice<R>(p.tab.ice, 0, ti);
+            const R t_qi_fall = apply_table_ice<R>(p.tab.ice, 1, ti);
+            const R t_lammax = apply_table_ice<R>(p.tab.ice, 6, ti);
+            const R t_lammin = apply_table_ice<R>(p.tab.ice, 7, ti);
+            nir = vmin(nir, t_lammax * nir);
+            nir = vmax(nir, t_lammin * nir);
+            q[1][t] = double(nir) * cfi;
+            const R rhofaci = R(coef[t]);
+            V1[t] = double(R(rt.ice_sedimentation_factor) * t_qi_fall * rhofaci);
+            V2[t] = double(R(rt.ice_sedimentation_factor) * t_ni_fall * rhofaci);
+          }
+        }
+        const double Co = V1[t] * dt_left * g.inv_dz[t];
+        if (Co > co) co = Co;
+      }
+    }
+    const double Co_max = grp_maxd<LPC>(co);
+
+    // generalized_sedimentation + calc_first_order_upwind_step, P3I/p3_upwind_impl.hpp:14-129
+    const int tmpint1 = (int)(Co_max + 1);
+    const double dt_sub = dt_left / tmpint1;
+    const int k_temp = (k_bot == kbot) ? k_bot : k_bot + 1;
+    // fluxes F = V r rho on [k_top, k_temp] (V = 0 under the window)
+    double F[NF][L];
+#pragma unroll
+    for (int t = 0; t < L; ++t) {
+      const int k = kbase + t;
+      const bool in = k >= k_top && k <= k_temp;
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        const double V = (SP == 2) ? (f == 1 ? V2[t] : V1[t]) : (f == 0 ? V1[t] : V2[t]);
+        F[f][t] = in ? V * q[f][t] * g.rho[t] : 0.0;
+      }
+    }
+    // flux out of the last level of the lane above
+    double Fin[NF];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      Fin[f] = __shfl_up(F[f][L - 1], 1, LPC);
+      if (j == 0) Fin[f] = 0;
+    }
+#pragma unroll
+    for (int t = 0; t < L; ++t) {
+      const int k = kbase + t;
+      if (k >= k_top && k <= k_temp) {
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+          if (f < nfield) {
+            const double Fa = (k == k_top) ? 0.0 : (t == 0 ? Fin[f] : F[f][t - 1]);
+            const double fluxdiv = (Fa - F[f][t]) * g.inv_dz[t];
+            q[f][t] = q[f][t] + fluxdiv * dt_sub * g.inv_rho[t];
+          }
+        }
+        if (SP == 1) pflux[t] += F[0][t];  // rain_sed :140-153, interface k+1
+        if (k == kbot && k_bot == kbot) prt_accum += F[0][t] * dt_sub;
+      }
+    }
+    if (k_bot != kbot) k_bot += 1;
+    dt_left -= dt_sub;
+    nsub++;
+    first = false;
+  }
+  if (present) {
+#pragma unroll
+    for (int t = 0; t < L; ++t) {
+      const int k = kbase + t;
+      if (k < nlev) {
+        const size_t x = (size_t)k * ld + i;
+#pragma unroll
+        for (int f = 0; f < NF; ++f) fld[f][x] = q[f][t];
+      }
+    }
+  }
+  prt = prt_accum;
+  nsub_out = nsub;
+}
+
+// One block's work: WPB wavefronts of 64/LPC columns each, starting at column
+// bid * WPB * 64/LPC of the column set p.
+template <class R, int LPC, int L, int WPB>
+__device__ __forceinline__ void sed_block(const KParams& p, int bid) {
+  constexpr int CPW = 64 / LPC;  // columns per wavefront
+  static_assert(64 % LPC == 0, "a column group must divide the wavefront");
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int j = lane % LPC;
+  const int i = (bid * WPB + wave) * CPW + lane / LPC;
+  // Whole groups leave together: the cross-lane operations below stay inside a group.
+  if (i >= p.ncol) return;
+  const int nlev = p.nlev, kbot = nlev - 1;
+  const int kbase = j * L;
+  const size_t ld = (size_t)p.ld;
+  const double dt = p.dt, inv_dt = 1 / dt;
+  const double latvap = LatVap, latice = LatIce, inv_cp = INV_CP;
+  const Runtime& rt = p.rt;
+  const bool active = p.colstat[i] == 2;
+  unsigned err = 0;
+
+  double pflux[L];
+#pragma unroll
+  for (int t = 0; t < L; ++t) pflux[t] = 0;
+  double prt_c = 0, prt_r = 0, prt_i = 0;
+  int ns_c = 0, ns_r = 0, ns_i = 0;
+  bool pres_c = false, pres_r = false, pres_i = false;
+
+  if (active) {
+    Geo<L> g;
+#pragma unroll
+    for (int t = 0; t < L; ++t) {
+      const int k = kbase + t;
+      const size_t x = (size_t)(k < nlev ? k : 0) * ld + i;
+      const double dz = k < nlev ? p.dz[x] : 1.0;
+      const double dp = k < nlev ? p.dpres[x] : 1.0;
+      g.rho[t] = dp / dz / gravit;
+      g.inv_rho[t] = 1 / g.rho[t];
+      g.inv_dz[t] = 1 / dz;
+    }
+    sediment<0, R, LPC, L>(p, i, kbase, g, pflux, prt_c, ns_c, pres_c, err);
+    sediment<1, R, LPC, L>(p, i, kbase, g, pflux, prt_r, ns_r, pres_r, err);
+    sediment<2, R, LPC, L>(p, i, kbase, g, pflux, prt_i, ns_i, pres_i, err);
+  }
+
+  // ---- homogeneous_freezing (P3I/p3_ice_sed_impl.hpp:207-270) + p3_main_part3
+#pragma unroll
+  for (int t = 0; t < L; ++t) {
+    const int k = kbase + t;
+    if (k >= nlev) continue;
+    const size_t x = (size_t)k * ld + i;
+    p.precip_liq_flux[x + ld] = pflux[t];
+    p.precip_ice_flux[x] = 0;
+    double reff_c = 10.e-6, reff_i = 25.e-6, reff_r = 500.e-6, rho_qi = 0;
+    if (active) {
+      double qc = p.qc[x], nc = p.nc[x], qr = p.qr[x], nr = p.nr[x], qi = p.qi[x], ni = p.ni[x];
+      double qm = p.qm[x], bm = p.bm[x], qv = p.qv[x], th = p.th[x];
+      const double inv_exner = p.inv_exner[x];
+      if (rt.do_ice_production) {
+        const bool t_lt = p.scratch[((size_t)SC_T * nlev + k) * ld + i] < T_homogfrz;
+        const bool qc_ge = t_lt && qc >= QSMALL;
+        const bool qr_ge = t_lt && qr >= QSMALL;
+        const double Qc_nuc = qc, Qr_nuc = qr;
+        const double Nc_nuc = vmax(nc, NSMALL), Nr_nuc = vmax(nr, NSMALL);
+        if (qc_ge) {
+          qm = qm + Qc_nuc; qi = qi + Qc_nuc; bm = bm + Qc_nuc * INV_RHO_RIMEMAX; ni = ni + Nc_nuc;
+          th = th + inv_exner * Qc_nuc * latice * inv_cp;
+        }
+        if (qr_ge) {
+          qm = qm + Qr_nuc; qi = qi + Qr_nuc; bm = bm + Qr_nuc * INV_RHO_RIMEMAX; ni = ni + Nr_nuc;
+          th = th + inv_exner * Qr_nuc * latice * inv_cp;
+        }
+        if (qc_ge) { qc = 0; nc = 0; }
+        if (qr_ge) { qr = 0; nr = 0; }
+      }
+      // p3_main_part3, P3I/p3_main_impl_part3.hpp:60-199 (only the fields p3_main outputs)
+      const double rho = p.dpres[x] / p.dz[x] / gravit;
+      const double inv_rho = 1 / rho;
+      const double cfl = p.cld_frac_l[x], cfr = p.cld_frac_r[x], cfi = p.cld_frac_i[x];
+      double vap_liq = p.vap_liq_exchange[x];
+      // cloud
+      if (qc >= QSMALL) {
+        R ncr = R(nc / cfl), mu_c, lamc, cd, cd1;
+        get_cloud_dsd2<R>(R(qc / cfl), ncr, mu_c, R(rho), lamc, cd, cd1);
+        nc = double(ncr) * cfl;
+        reff_c = double(R(0.5) * (mu_c + R(3)) / lamc);
+      } else {
+        qv = qv + qc;
+        th = th - inv_exner * qc * latvap * inv_cp;
+        vap_liq = vap_liq - qc;
+        qc = 0; nc = 0;
+      }
+      // rain
+      if (qr >= QSMALL) {
+        R nrr = R(nr / cfr), mu_r = 0, lamr;
+        get_rain_dsd2<R>(rt, R(qr / cfr), nrr, mu_r, lamr);
+        nr = double(nrr) * cfr;
+        reff_r = double(R(1.5) / lamr);
+      } else {
+        qv = qv + qr;
+        th = th - inv_exner * qr * latvap * inv_cp;
+        vap_liq = vap_liq - qr;
+        qr = 0; nr = 0;
+      }
+      // ice
+      {
+        const bool qi_gt = qi >= QSMALL;
+        ni = vmax(ni, NSMALL);
+        const double qi_incld = qi / cfi;
+        double ni_incld = ni / cfi;
+        R qmr = R(qm / cfi), bmr = R(bm / cfi);
+        double qm_incld = qm / cfi, bm_incld = bm / cfi;
+        R rhop = 0;
+        if (qi_gt) {
+          rhop = calc_bulk_rho_rime<R>(rt, R(qi_incld), qmr, bmr);
+          qm_incld = double(qmr); bm_incld = double(bmr);
+          qm = qm_incld * cfi;
+          bm = bm_incld * cfi;
+        }
+        impose_max_total_ni<double>(ni_incld, rt.max_total_ni, inv_rho);
+        if (qi_gt) {
+          const TabIce<R> ti = lookup_ice<R>(R(qi_incld), R(ni_incld), R(qm_incld), rhop);
+          if (p.dbg_ice_idx) p.dbg_ice_idx[x] = ti.dumjj * 10000 + ti.dumii * 100 + ti.dumi;
+          const R t_eff = apply_table_ice<R>(p.tab.ice, 5, ti);
+          const R t_lammax = apply_table_ice<R>(p.tab.ice, 6, ti);
+          const R t_lammin = apply_table_ice<R>(p.tab.ice, 7, ti);
+          const R t_rhomm = apply_table_ice<R>(p.tab.ice, 11, ti);
+          ni_incld = vmin(ni_incld, double(t_lammax) * ni_incld);
+          ni_incld = vmax(ni_incld, double(t_lammin) * ni_incld);
+          reff_i = double(t_eff);
+          rho_qi = double(t_rhomm);
+        } else if (p.dbg_ice_idx) {
+          p.dbg_ice_idx[x] = -1;
+        }
+        ni = ni_incld * cfi;
+        if (qm < QSMALL && qi_gt) { qm = 0; bm = 0; }
+        if (!qi_gt) {
+          qv = qv + qi;
+          th = th - inv_exner * qi * (latvap + latice) * inv_cp;
+          qi = 0; ni = 0; qm = 0; bm = 0;
+        }
+      }
+      if (qr < QSMALL) nr = 0;
+      p.qc[x] = qc; p.nc[x] = nc; p.qr[x] = qr; p.nr[x] = nr; p.qi[x] = qi;
+      p.ni[x] = ni; p.qm[x] = qm; p.bm[x] = bm; p.qv[x] = qv; p.th[x] = th;
+      p.vap_liq_exchange[x] = vap_liq;
+    } else if (p.dbg_ice_idx) {
+      p.dbg_ice_idx[x] = -1;
+    }
+    p.diag_eff_radius_qc[x] = reff_c;
+    p.diag_eff_radius_qi[x] = reff_i;
+    p.diag_eff_radius_qr[x] = reff_r;
+    p.rho_qi[x] = rho_qi;
+    if (k == kbot) {
+      p.precip_ice_flux[x + ld] = 0;
+      double liq = 0, ice = 0;
+      if (pres_c) liq = prt_c * INV_RHO_H2O * inv_dt;
+      if (pres_r) liq += prt_r * INV_RHO_H2O * inv_dt;
+      if (pres_i) ice += prt_i * INV_RHO_H2O * inv_dt;
+      p.precip_liq_surf[i] = liq;
+      p.precip_ice_surf[i] = ice;
+    }
+  }
+  if (j == 0) {
+    p.precip_liq_flux[i] = 0;  // the top interface
+    if (p.dbg_substeps) {
+      p.dbg_substeps[i] = ns_c;
+      p.dbg_substeps[(size_t)p.ncol + i] = ns_r;
+      p.dbg_substeps[(size_t)2 * p.ncol + i] = ns_i;
+    }
+  }
+  if (err) atomicOr(p.err, err);
+}
+
+template <class R, int LPC, int L, int WPB>
+__global__ __launch_bounds__(64 * WPB) void p3_sed_kernel(const KParams p) {
+  sed_block<R, LPC, L, WPB>(p, blockIdx.x);
+}
+
+}  // namespace p3d

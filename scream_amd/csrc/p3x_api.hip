@@ -2,20 +2,23 @@
 //
 // A context owns the device copies of the lookup tables (P3LookupTables,
 // components/eamxx/src/physics/p3/p3_functions.hpp:279-290), the level-major
-// scratch the fused kernel keeps between phases (the counterpart of the
-// reference's WorkspaceManager, eamxx_p3_process_interface.cpp:423-424), an
-// error word and timing events.  Nothing is allocated per call unless a
-// larger problem than any before needs more scratch.
+// scratch the two kernels of a step share (the counterpart of the reference's
+// WorkspaceManager, eamxx_p3_process_interface.cpp:423-424), an error word and
+// timing events.  Nothing is allocated per call unless a larger problem than
+// any before needs more scratch.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <vector>
 
 #include "../../include/p3x.h"
 #include "p3_launch.hpp"
 
 struct p3x_context {
   int device = 0;
+  int ncu = 0;
   double* d_ice = nullptr;
   double* d_coll = nullptr;
   double* d_vn = nullptr;
@@ -33,13 +36,72 @@ namespace {
 
 int dev_fail(hipError_t e) { return e == hipErrorOutOfMemory ? P3X_ERR_NO_MEMORY : P3X_ERR_DEVICE; }
 
-// Tuned default block shape per (precision, nlev); see the autotuner in
-// scream_amd/tune.py for how these were chosen.
-void default_shape(int precision, int nk, int* cb, int* nt) {
+// Default launch shape per (precision, nlev).
+void default_shape(int precision, int nk, p3d::Shape* sh) {
   (void)precision;
-  (void)nk;
-  *cb = 32;
-  *nt = 256;
+  sh->cb = 32;
+  sh->nt = 256;
+  sh->lpc = nk <= 32 * p3d::kMaxLevelsPerLane ? 32 : 64;
+}
+
+// KParams of one column set; scratch points at SC_COUNT*nk*ld doubles followed
+// by ld ints.
+void fill_params(const p3x_context* c, const p3x_runtime* runtime, const p3x_prognostic_state* ps,
+                 const p3x_diagnostic_inputs* di, const p3x_diagnostic_outputs* dout,
+                 const p3x_infrastructure* infra, const p3x_history_only* hist, int nj, int nk, int ld,
+                 int* col_status, int* sed_substeps, int* ice_table_idx, double* scratch,
+                 p3d::KParams* out) {
+  p3d::KParams& k = *out;
+  k.qc = ps->qc; k.nc = ps->nc; k.qr = ps->qr; k.nr = ps->nr; k.qi = ps->qi;
+  k.qm = ps->qm; k.ni = ps->ni; k.bm = ps->bm; k.qv = ps->qv; k.th = ps->th;
+  k.nc_nuceat_tend = di->nc_nuceat_tend; k.nccn = di->nccn; k.ni_activated = di->ni_activated;
+  k.inv_qc_relvar = di->inv_qc_relvar; k.cld_frac_i = di->cld_frac_i; k.cld_frac_l = di->cld_frac_l;
+  k.cld_frac_r = di->cld_frac_r; k.pres = di->pres; k.dz = di->dz; k.dpres = di->dpres;
+  k.inv_exner = di->inv_exner; k.qv_prev = di->qv_prev; k.t_prev = di->t_prev;
+  k.qv2qi_depos_tend = dout->qv2qi_depos_tend; k.precip_liq_surf = dout->precip_liq_surf;
+  k.precip_ice_surf = dout->precip_ice_surf; k.diag_eff_radius_qc = dout->diag_eff_radius_qc;
+  k.diag_eff_radius_qi = dout->diag_eff_radius_qi; k.diag_eff_radius_qr = dout->diag_eff_radius_qr;
+  k.rho_qi = dout->rho_qi; k.precip_liq_flux = dout->precip_liq_flux; k.precip_ice_flux = dout->precip_ice_flux;
+  k.precip_total_tend = dout->precip_total_tend; k.nevapr = dout->nevapr;
+  k.liq_ice_exchange = hist->liq_ice_exchange; k.vap_liq_exchange = hist->vap_liq_exchange;
+  k.vap_ice_exchange = hist->vap_ice_exchange;
+  k.scratch = scratch;
+  k.colstat = (int*)(scratch + (size_t)p3d::SC_COUNT * nk * ld);
+  k.err = c->d_err;
+  k.dbg_status = col_status;
+  k.dbg_substeps = sed_substeps;
+  k.dbg_ice_idx = ice_table_idx;
+  k.tab.ice = c->d_ice; k.tab.coll = c->d_coll; k.tab.vn = c->d_vn; k.tab.vm = c->d_vm; k.tab.revap = c->d_revap;
+  static_assert(sizeof(p3d::Runtime) == sizeof(p3x_runtime), "runtime layouts differ");
+  std::memcpy(&k.rt, runtime, sizeof(k.rt));
+  k.ncol = nj; k.nlev = nk; k.ld = ld;
+  k.dt = infra->dt;
+  k.predictNc = infra->predictNc ? 1 : 0;
+  k.prescribedCCN = infra->prescribedCCN ? 1 : 0;
+  k.exch_zero = infra->exchange_zeroed_on_entry ? 1 : 0;
+}
+
+// Every field pointer of a column set is non-null.
+bool fields_present(const p3x_prognostic_state* ps, const p3x_diagnostic_inputs* di,
+                    const p3x_diagnostic_outputs* dout, const p3x_history_only* hist) {
+  const void* must[] = {ps->qc, ps->nc, ps->qr, ps->nr, ps->qi, ps->qm, ps->ni, ps->bm, ps->qv, ps->th,
+                        di->nc_nuceat_tend, di->nccn, di->ni_activated, di->inv_qc_relvar, di->cld_frac_i,
+                        di->cld_frac_l, di->cld_frac_r, di->pres, di->dz, di->dpres, di->inv_exner,
+                        di->qv_prev, di->t_prev, dout->qv2qi_depos_tend, dout->precip_liq_surf,
+                        dout->precip_ice_surf, dout->diag_eff_radius_qc, dout->diag_eff_radius_qi,
+                        dout->diag_eff_radius_qr, dout->rho_qi, dout->precip_liq_flux, dout->precip_ice_flux,
+                        dout->precip_total_tend, dout->nevapr, hist->liq_ice_exchange,
+                        hist->vap_liq_exchange, hist->vap_ice_exchange};
+  for (const void* p : must)
+    if (!p) return false;
+  return true;
+}
+
+// Bytes of scratch one column set needs (SC_COUNT level-major fields, then ld
+// column statuses), rounded up to 256.
+size_t scratch_bytes_for(int nk, int ld) {
+  const size_t b = (size_t)p3d::SC_COUNT * nk * ld * sizeof(double) + (size_t)ld * sizeof(int);
+  return (b + 255) & ~(size_t)255;
 }
 
 }  // namespace
@@ -107,6 +169,7 @@ extern "C" int p3x_init(const p3x_table_data* t, p3x_context** out) {
   if (e == hipSuccess) e = hipMemset(c->d_err, 0, sizeof(unsigned));
   if (e == hipSuccess) e = hipEventCreate(&c->ev0);
   if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, c->device);
   if (e != hipSuccess) {
     p3x_finalize(c);
     return dev_fail(e);
@@ -157,24 +220,19 @@ extern "C" int p3x_main(p3x_context* c, const p3x_runtime* runtime, const p3x_pr
   const int ld = o.ld > 0 ? o.ld : nj;
   if (ld < nj) return P3X_ERR_ARG;
   if (o.precision != P3X_FP64 && o.precision != P3X_MIXED) return P3X_ERR_ARG;
-  int cb = o.columns_per_block, nt = o.threads_per_block;
-  if (cb == 0 || nt == 0) default_shape(o.precision, nk, &cb, &nt);
+  if (nk > 64 * p3d::kMaxLevelsPerLane) return P3X_ERR_ARG;
+  p3d::Shape sh;
+  default_shape(o.precision, nk, &sh);
+  if (o.columns_per_block != 0 || o.threads_per_block != 0) {
+    sh.cb = o.columns_per_block;
+    sh.nt = o.threads_per_block;
+  }
+  if (o.sed_lanes_per_column != 0) sh.lpc = o.sed_lanes_per_column;
   hipStream_t s = (hipStream_t)o.stream;
   if (nj == 0) return P3X_OK;
+  if (!fields_present(ps, di, dout, hist)) return P3X_ERR_ARG;
 
-  const void* must[] = {ps->qc, ps->nc, ps->qr, ps->nr, ps->qi, ps->qm, ps->ni, ps->bm, ps->qv, ps->th,
-                        di->nc_nuceat_tend, di->nccn, di->ni_activated, di->inv_qc_relvar, di->cld_frac_i,
-                        di->cld_frac_l, di->cld_frac_r, di->pres, di->dz, di->dpres, di->inv_exner,
-                        di->qv_prev, di->t_prev, dout->qv2qi_depos_tend, dout->precip_liq_surf,
-                        dout->precip_ice_surf, dout->diag_eff_radius_qc, dout->diag_eff_radius_qi,
-                        dout->diag_eff_radius_qr, dout->rho_qi, dout->precip_liq_flux, dout->precip_ice_flux,
-                        dout->precip_total_tend, dout->nevapr, hist->liq_ice_exchange,
-                        hist->vap_liq_exchange, hist->vap_ice_exchange};
-  for (const void* p : must)
-    if (!p) return P3X_ERR_ARG;
-
-  // scratch: SC_COUNT level-major fields of nk x ld doubles
-  const size_t need = (size_t)p3d::SC_COUNT * nk * ld * sizeof(double);
+  const size_t need = scratch_bytes_for(nk, ld);
   if (need > c->scratch_bytes) {
     hipError_t e = hipStreamSynchronize(c->last_stream);
     if (e == hipSuccess && c->d_scratch) e = hipFree(c->d_scratch);
@@ -186,36 +244,12 @@ extern "C" int p3x_main(p3x_context* c, const p3x_runtime* runtime, const p3x_pr
   }
 
   p3d::KParams k;
-  k.qc = ps->qc; k.nc = ps->nc; k.qr = ps->qr; k.nr = ps->nr; k.qi = ps->qi;
-  k.qm = ps->qm; k.ni = ps->ni; k.bm = ps->bm; k.qv = ps->qv; k.th = ps->th;
-  k.nc_nuceat_tend = di->nc_nuceat_tend; k.nccn = di->nccn; k.ni_activated = di->ni_activated;
-  k.inv_qc_relvar = di->inv_qc_relvar; k.cld_frac_i = di->cld_frac_i; k.cld_frac_l = di->cld_frac_l;
-  k.cld_frac_r = di->cld_frac_r; k.pres = di->pres; k.dz = di->dz; k.dpres = di->dpres;
-  k.inv_exner = di->inv_exner; k.qv_prev = di->qv_prev; k.t_prev = di->t_prev;
-  k.qv2qi_depos_tend = dout->qv2qi_depos_tend; k.precip_liq_surf = dout->precip_liq_surf;
-  k.precip_ice_surf = dout->precip_ice_surf; k.diag_eff_radius_qc = dout->diag_eff_radius_qc;
-  k.diag_eff_radius_qi = dout->diag_eff_radius_qi; k.diag_eff_radius_qr = dout->diag_eff_radius_qr;
-  k.rho_qi = dout->rho_qi; k.precip_liq_flux = dout->precip_liq_flux; k.precip_ice_flux = dout->precip_ice_flux;
-  k.precip_total_tend = dout->precip_total_tend; k.nevapr = dout->nevapr;
-  k.liq_ice_exchange = hist->liq_ice_exchange; k.vap_liq_exchange = hist->vap_liq_exchange;
-  k.vap_ice_exchange = hist->vap_ice_exchange;
-  k.scratch = c->d_scratch;
-  k.err = c->d_err;
-  k.dbg_status = o.col_status;
-  k.dbg_substeps = o.sed_substeps;
-  k.dbg_ice_idx = o.ice_table_idx;
-  k.tab.ice = c->d_ice; k.tab.coll = c->d_coll; k.tab.vn = c->d_vn; k.tab.vm = c->d_vm; k.tab.revap = c->d_revap;
-  static_assert(sizeof(p3d::Runtime) == sizeof(p3x_runtime), "runtime layouts differ");
-  std::memcpy(&k.rt, runtime, sizeof(k.rt));
-  k.ncol = nj; k.nlev = nk; k.ld = ld;
-  k.dt = infra->dt;
-  k.predictNc = infra->predictNc ? 1 : 0;
-  k.prescribedCCN = infra->prescribedCCN ? 1 : 0;
-  k.exch_zero = infra->exchange_zeroed_on_entry ? 1 : 0;
+  fill_params(c, runtime, ps, di, dout, infra, hist, nj, nk, ld, o.col_status, o.sed_substeps,
+              o.ice_table_idx, c->d_scratch, &k);
 
   hipError_t e = hipEventRecord(c->ev0, s);
   if (e != hipSuccess) return P3X_ERR_DEVICE;
-  e = o.precision == P3X_FP64 ? p3d::launch_p3_fp64(k, cb, nt, s) : p3d::launch_p3_mixed(k, cb, nt, s);
+  e = o.precision == P3X_FP64 ? p3d::launch_p3_fp64(k, sh, s) : p3d::launch_p3_mixed(k, sh, s);
   if (e == hipErrorInvalidValue) return P3X_ERR_ARG;
   if (e != hipSuccess) return P3X_ERR_DEVICE;
   e = hipEventRecord(c->ev1, s);
@@ -228,4 +262,331 @@ extern "C" int p3x_main(p3x_context* c, const p3x_runtime* runtime, const p3x_pr
     if (hipEventElapsedTime(&ms, c->ev0, c->ev1) == hipSuccess) *elapsed_us = 1000.0 * ms;
   }
   return st;
+}
+
+// ============================================================ batched form
+
+struct p3x_batch {
+  p3x_context* ctx = nullptr;
+  p3x_batch_options opt{};
+  std::vector<p3d::KParams> sets;  // in the caller's order
+  double* d_scratch = nullptr;
+  p3d::KParams* d_sets = nullptr;  // caller's order (part 1 + part 2 launch)
+  int* d_first = nullptr;
+  p3d::KParams* d_sets_sed = nullptr;  // grouped by sedimentation class
+  int* d_first_sed = nullptr;
+  p3d::BatchPlan plan{};
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+void free_plan(p3x_batch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  (void)hipFree(b->d_first);
+  (void)hipFree(b->d_sets_sed);
+  (void)hipFree(b->d_first_sed);
+  b->d_first = nullptr;
+  b->d_sets_sed = nullptr;
+  b->d_first_sed = nullptr;
+}
+
+hipError_t launch_batch(const p3x_batch* b, hipStream_t s) {
+  return b->opt.precision == P3X_FP64 ? p3d::launch_p3_batch_fp64(b->plan, s)
+                                      : p3d::launch_p3_batch_mixed(b->plan, s);
+}
+
+// Lay the sets out for b->opt: virtual-block offsets of both launches, the
+// sedimentation classes, persistent grid sizes and (use_graph) the graph.
+int build_plan(p3x_batch* b) {
+  p3x_context* c = b->ctx;
+  const int n = (int)b->sets.size();
+  hipError_t e = hipDeviceSynchronize();
+  if (e != hipSuccess) return P3X_ERR_DEVICE;
+  free_plan(b);
+  int nk_max = 1;
+  for (const auto& k : b->sets) nk_max = std::max(nk_max, k.nlev);
+  p3d::Shape sh;
+  default_shape(b->opt.precision, nk_max, &sh);
+  if (b->opt.columns_per_block != 0 || b->opt.threads_per_block != 0) {
+    sh.cb = b->opt.columns_per_block;
+    sh.nt = b->opt.threads_per_block;
+  }
+  if (b->opt.sed_lanes_per_column != 0) sh.lpc = b->opt.sed_lanes_per_column;
+  if (sh.lpc != 16 && sh.lpc != 32 && sh.lpc != 64) return P3X_ERR_ARG;
+  b->plan = p3d::BatchPlan{};
+  b->plan.shape = sh;
+
+  // part 1 + part 2: every set, caller's order
+  std::vector<int> first(n + 1, 0);
+  for (int i = 0; i < n; ++i) first[i + 1] = first[i] + (b->sets[i].ncol + sh.cb - 1) / sh.cb;
+  // sedimentation: classes of equal levels-per-lane
+  const int cols_sed = p3d::kSedWaves * (64 / sh.lpc);
+  std::vector<int> cls_of(n);
+  for (int i = 0; i < n; ++i) {
+    cls_of[i] = p3d::sed_levels_per_lane(b->sets[i].nlev, sh.lpc);
+    if (cls_of[i] == 0) return P3X_ERR_ARG;
+  }
+  std::vector<p3d::KParams> sed_sets;
+  std::vector<int> sed_first;
+  std::vector<std::pair<int, int>> cls_span;  // (offset in sed_sets, offset in sed_first)
+  std::vector<int> cls_L;
+  for (int L = 1; L <= p3d::kMaxLevelsPerLane; ++L) {
+    const size_t s0 = sed_sets.size(), f0 = sed_first.size();
+    int acc = 0;
+    for (int i = 0; i < n; ++i) {
+      if (cls_of[i] != L) continue;
+      if (sed_sets.size() == s0) sed_first.push_back(0);
+      sed_sets.push_back(b->sets[i]);
+      acc += (b->sets[i].ncol + cols_sed - 1) / cols_sed;
+      sed_first.push_back(acc);
+    }
+    if (sed_sets.size() > s0) {
+      cls_span.push_back({(int)s0, (int)f0});
+      cls_L.push_back(L);
+    }
+  }
+  e = hipMalloc((void**)&b->d_first, first.size() * sizeof(int));
+  if (e == hipSuccess) e = hipMemcpy(b->d_first, first.data(), first.size() * sizeof(int), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMalloc((void**)&b->d_sets_sed, sed_sets.size() * sizeof(p3d::KParams));
+  if (e == hipSuccess)
+    e = hipMemcpy(b->d_sets_sed, sed_sets.data(), sed_sets.size() * sizeof(p3d::KParams), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMalloc((void**)&b->d_first_sed, sed_first.size() * sizeof(int));
+  if (e == hipSuccess)
+    e = hipMemcpy(b->d_first_sed, sed_first.data(), sed_first.size() * sizeof(int), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return dev_fail(e);
+
+  auto grid_for = [&](int total, int resident) {
+    long g = b->opt.blocks_per_cu < 0 ? total
+             : (long)c->ncu * (b->opt.blocks_per_cu > 0 ? b->opt.blocks_per_cu : std::max(1, resident));
+    return (int)std::max(1L, std::min<long>(g, total));
+  };
+  b->plan.sets = b->d_sets;
+  b->plan.first = b->d_first;
+  b->plan.nsets = n;
+  b->plan.nclass = (int)cls_L.size();
+  for (int ci = 0; ci < b->plan.nclass; ++ci) {
+    p3d::SedClass& sc = b->plan.cls[ci];
+    sc.L = cls_L[ci];
+    sc.sets = b->d_sets_sed + cls_span[ci].first;
+    sc.first = b->d_first_sed + cls_span[ci].second;
+    const size_t next_s = ci + 1 < b->plan.nclass ? cls_span[ci + 1].first : sed_sets.size();
+    sc.nsets = (int)(next_s - cls_span[ci].first);
+    int r12 = 1, rsed = 1;
+    e = b->opt.precision == P3X_FP64 ? p3d::occupancy_p3_fp64(sh, sc.L, &r12, &rsed)
+                                      : p3d::occupancy_p3_mixed(sh, sc.L, &r12, &rsed);
+    if (e == hipErrorInvalidValue) return P3X_ERR_ARG;
+    if (e != hipSuccess) return P3X_ERR_DEVICE;
+    sc.grid = grid_for(sed_first[cls_span[ci].second + sc.nsets], rsed);
+    if (ci == 0) b->plan.grid = grid_for(first[n], r12);
+  }
+
+  if (b->opt.use_graph) {
+    hipStream_t cs = nullptr;
+    e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed);
+    if (e == hipSuccess) {
+      const hipError_t le = launch_batch(b, cs);
+      e = hipStreamEndCapture(cs, &b->graph);
+      if (le != hipSuccess) e = le;
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+    if (cs) (void)hipStreamDestroy(cs);
+    if (e == hipErrorInvalidValue) return P3X_ERR_ARG;
+    if (e != hipSuccess) return dev_fail(e);
+  }
+  return P3X_OK;
+}
+
+// The level-major arrays p3_main updates in place (state, and the exchange
+// fields unless the set declares them zeroed on entry), for the autotuner's
+// save/restore.
+void inout_arrays(const p3d::KParams& k, std::vector<double*>* out) {
+  double* f[] = {k.qc, k.nc, k.qr, k.nr, k.qi, k.qm, k.ni, k.bm, k.qv, k.th};
+  for (double* p : f) out->push_back(p);
+  if (!k.exch_zero) {
+    out->push_back(k.liq_ice_exchange);
+    out->push_back(k.vap_liq_exchange);
+    out->push_back(k.vap_ice_exchange);
+  }
+}
+
+}  // namespace
+
+extern "C" void p3x_batch_defaults(p3x_batch_options* o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->precision = P3X_MIXED;
+  o->use_graph = 1;
+}
+
+extern "C" int p3x_batch_create(p3x_context* c, const p3x_runtime* runtime, const p3x_column_set* sets,
+                                int nsets, const p3x_batch_options* opts, p3x_batch** out) {
+  if (!c || !runtime || !out || nsets <= 0 || !sets) return P3X_ERR_ARG;
+  *out = nullptr;
+  p3x_batch_options o;
+  if (opts) o = *opts;
+  else p3x_batch_defaults(&o);
+  if (o.precision != P3X_FP64 && o.precision != P3X_MIXED) return P3X_ERR_ARG;
+  std::vector<size_t> offs(nsets);
+  size_t total = 0;
+  for (int i = 0; i < nsets; ++i) {
+    const p3x_column_set& s = sets[i];
+    const int ld = s.ld > 0 ? s.ld : s.nj;
+    if (s.nj <= 0 || s.nk <= 0 || ld < s.nj || s.nk > 64 * p3d::kMaxLevelsPerLane) return P3X_ERR_ARG;
+    if (!(s.infrastructure.dt > 0)) return P3X_ERR_ARG;
+    if (!fields_present(&s.prognostic_state, &s.diagnostic_inputs, &s.diagnostic_outputs, &s.history_only))
+      return P3X_ERR_ARG;
+    offs[i] = total;
+    total += scratch_bytes_for(s.nk, ld);
+  }
+  p3x_batch* b = new p3x_batch();
+  b->ctx = c;
+  b->opt = o;
+  hipError_t e = hipMalloc((void**)&b->d_scratch, total);
+  if (e == hipSuccess) {
+    b->sets.resize(nsets);
+    for (int i = 0; i < nsets; ++i) {
+      const p3x_column_set& s = sets[i];
+      const int ld = s.ld > 0 ? s.ld : s.nj;
+      fill_params(c, runtime, &s.prognostic_state, &s.diagnostic_inputs, &s.diagnostic_outputs,
+                  &s.infrastructure, &s.history_only, s.nj, s.nk, ld, s.col_status, s.sed_substeps,
+                  s.ice_table_idx, (double*)((char*)b->d_scratch + offs[i]), &b->sets[i]);
+    }
+    e = hipMalloc((void**)&b->d_sets, nsets * sizeof(p3d::KParams));
+  }
+  if (e == hipSuccess) e = hipMemcpy(b->d_sets, b->sets.data(), nsets * sizeof(p3d::KParams), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    p3x_batch_destroy(b);
+    return dev_fail(e);
+  }
+  const int st = build_plan(b);
+  if (st != P3X_OK) {
+    p3x_batch_destroy(b);
+    return st;
+  }
+  *out = b;
+  return P3X_OK;
+}
+
+extern "C" int p3x_batch_run(p3x_batch* b, void* stream) {
+  if (!b) return P3X_ERR_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  const hipError_t e = b->exec ? hipGraphLaunch(b->exec, s) : launch_batch(b, s);
+  if (e == hipErrorInvalidValue) return P3X_ERR_ARG;
+  if (e != hipSuccess) return P3X_ERR_DEVICE;
+  b->ctx->last_stream = s;
+  return P3X_OK;
+}
+
+extern "C" int p3x_batch_tune(p3x_batch* b, void* stream, int reps, p3x_batch_options* chosen,
+                              double* best_us) {
+  if (!b || reps <= 0) return P3X_ERR_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  // save the in/out arrays so that every trial starts from the same state
+  std::vector<double*> arrs;
+  std::vector<size_t> bytes;
+  for (const auto& k : b->sets) {
+    const size_t n0 = arrs.size();
+    inout_arrays(k, &arrs);
+    for (size_t a = n0; a < arrs.size(); ++a) bytes.push_back((size_t)k.nlev * k.ld * sizeof(double));
+  }
+  size_t total = 0;
+  for (size_t x : bytes) total += x;
+  char* backup = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipMalloc((void**)&backup, total);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  auto copy_all = [&](bool save) {
+    size_t off = 0;
+    for (size_t a = 0; a < arrs.size() && e == hipSuccess; ++a) {
+      e = save ? hipMemcpyAsync(backup + off, arrs[a], bytes[a], hipMemcpyDeviceToDevice, s)
+               : hipMemcpyAsync(arrs[a], backup + off, bytes[a], hipMemcpyDeviceToDevice, s);
+      off += bytes[a];
+    }
+  };
+  copy_all(true);
+  int st = P3X_OK;
+  // mean device microseconds of one step with b->opt (state restored around it)
+  auto trial = [&](double* us) {
+    st = build_plan(b);
+    if (st != P3X_OK) return;
+    copy_all(false);
+    if (e == hipSuccess) st = p3x_batch_run(b, s);  // warm-up
+    double sum = 0;
+    for (int r = 0; r < reps && st == P3X_OK && e == hipSuccess; ++r) {
+      copy_all(false);
+      if (e == hipSuccess) e = hipEventRecord(e0, s);
+      if (e == hipSuccess) st = p3x_batch_run(b, s);
+      if (e == hipSuccess && st == P3X_OK) e = hipEventRecord(e1, s);
+      if (e == hipSuccess && st == P3X_OK) e = hipEventSynchronize(e1);
+      float ms = 0;
+      if (e == hipSuccess && st == P3X_OK) e = hipEventElapsedTime(&ms, e0, e1);
+      sum += 1000.0 * ms;
+    }
+    *us = sum / reps;
+  };
+  p3x_batch_options best = b->opt;
+  double best_t = 0;
+  if (e == hipSuccess) trial(&best_t);
+  static const int shapes[][2] = {{16, 128}, {16, 256}, {32, 256}, {64, 256}, {32, 512}, {64, 512}};
+  static const int lanes[] = {16, 32, 64};
+  static const int bpcs[] = {0, -1, 1, 2, 4};
+  auto consider = [&](const p3x_batch_options& cand) {
+    if (e != hipSuccess || st != P3X_OK) return;
+    b->opt = cand;
+    double t = 0;
+    trial(&t);
+    if (st == P3X_ERR_ARG) {  // shape not usable for these sets: skip it
+      st = P3X_OK;
+      return;
+    }
+    if (e == hipSuccess && st == P3X_OK && t < best_t) {
+      best_t = t;
+      best = cand;
+    }
+  };
+  for (const auto& sh : shapes) {
+    p3x_batch_options cand = best;
+    cand.columns_per_block = sh[0];
+    cand.threads_per_block = sh[1];
+    consider(cand);
+  }
+  for (int l : lanes) {
+    p3x_batch_options cand = best;
+    cand.sed_lanes_per_column = l;
+    consider(cand);
+  }
+  for (int g : bpcs) {
+    p3x_batch_options cand = best;
+    cand.blocks_per_cu = g;
+    consider(cand);
+  }
+  b->opt = best;
+  if (e == hipSuccess && st == P3X_OK) st = build_plan(b);
+  copy_all(false);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(backup);
+  b->ctx->last_stream = s;
+  if (e != hipSuccess) return dev_fail(e);
+  if (st != P3X_OK) return st;
+  if (chosen) *chosen = best;
+  if (best_us) *best_us = best_t;
+  return p3x_sync(b->ctx);
+}
+
+extern "C" void p3x_batch_destroy(p3x_batch* b) {
+  if (!b) return;
+  (void)hipDeviceSynchronize();
+  free_plan(b);
+  (void)hipFree(b->d_sets);
+  (void)hipFree(b->d_scratch);
+  delete b;
 }

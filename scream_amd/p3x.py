@@ -37,7 +37,8 @@ ALL_FIELDS = STATE + INPUTS + OUTS + HIST
 
 # Every symbol include/p3x.h declares.
 EXPORTS = ("p3x_runtime_defaults", "p3x_launch_defaults", "p3x_tables_load", "p3x_init",
-           "p3x_finalize", "p3x_main", "p3x_sync", "p3x_strerror")
+           "p3x_finalize", "p3x_main", "p3x_sync", "p3x_strerror", "p3x_batch_defaults",
+           "p3x_batch_create", "p3x_batch_run", "p3x_batch_tune", "p3x_batch_destroy")
 
 
 class Runtime(C.Structure):
@@ -76,9 +77,25 @@ class TableData(C.Structure):
 
 class Launch(C.Structure):
     _fields_ = [("precision", C.c_int), ("ld", C.c_int), ("columns_per_block", C.c_int),
-                ("threads_per_block", C.c_int), ("stream", C.c_void_p), ("async_", C.c_int),
+                ("threads_per_block", C.c_int), ("sed_lanes_per_column", C.c_int),
+                ("stream", C.c_void_p), ("async_", C.c_int),
                 ("col_status", C.c_void_p), ("sed_substeps", C.c_void_p),
                 ("ice_table_idx", C.c_void_p)]
+
+
+class ColumnSet(C.Structure):
+    _fields_ = [("prognostic_state", PrognosticState), ("diagnostic_inputs", DiagnosticInputs),
+                ("diagnostic_outputs", DiagnosticOutputs), ("history_only", HistoryOnly),
+                ("infrastructure", Infrastructure), ("nj", C.c_int), ("nk", C.c_int), ("ld", C.c_int),
+                ("col_status", C.c_void_p), ("sed_substeps", C.c_void_p), ("ice_table_idx", C.c_void_p)]
+
+
+class BatchOptions(C.Structure):
+    _fields_ = [("precision", C.c_int), ("columns_per_block", C.c_int), ("threads_per_block", C.c_int),
+                ("sed_lanes_per_column", C.c_int), ("blocks_per_cu", C.c_int), ("use_graph", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class P3Error(RuntimeError):
@@ -120,6 +137,16 @@ def load() -> C.CDLL:
         lib.p3x_sync.restype = C.c_int
         lib.p3x_strerror.argtypes = [C.c_int]
         lib.p3x_strerror.restype = C.c_char_p
+        lib.p3x_batch_defaults.argtypes = [C.POINTER(BatchOptions)]
+        lib.p3x_batch_create.argtypes = [C.c_void_p, C.POINTER(Runtime), C.POINTER(ColumnSet), C.c_int,
+                                         C.POINTER(BatchOptions), C.POINTER(C.c_void_p)]
+        lib.p3x_batch_create.restype = C.c_int
+        lib.p3x_batch_run.argtypes = [C.c_void_p, C.c_void_p]
+        lib.p3x_batch_run.restype = C.c_int
+        lib.p3x_batch_tune.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.POINTER(BatchOptions),
+                                       C.POINTER(C.c_double)]
+        lib.p3x_batch_tune.restype = C.c_int
+        lib.p3x_batch_destroy.argtypes = [C.c_void_p]
         _lib = lib
     return _lib
 
@@ -177,7 +204,8 @@ class Context:
 
     def main(self, f: dict, nj: int, nk: int, dt: float, predict_nc=True, prescribed_ccn=False,
              runtime: Runtime | None = None, precision: int = P3X_FP64, ld: int = 0,
-             columns_per_block: int = 0, threads_per_block: int = 0, stream=None,
+             columns_per_block: int = 0, threads_per_block: int = 0,
+             sed_lanes_per_column: int = 0, stream=None,
              async_: bool = False, debug: dict | None = None, exchange_zeroed=False,
              check: bool = True) -> float:
         """One p3x_main call; ``f`` maps every field name to a device array.
@@ -193,7 +221,8 @@ class Context:
                              exchange_zeroed_on_entry=int(bool(exchange_zeroed)))
         dbg = debug or {}
         lo = Launch(precision=int(precision), ld=int(ld), columns_per_block=int(columns_per_block),
-                    threads_per_block=int(threads_per_block), stream=stream,
+                    threads_per_block=int(threads_per_block),
+                    sed_lanes_per_column=int(sed_lanes_per_column), stream=stream,
                     async_=int(bool(async_)), col_status=_addr(dbg.get("col_status")),
                     sed_substeps=_addr(dbg.get("sed_substeps")),
                     ice_table_idx=_addr(dbg.get("ice_table_idx")))
@@ -207,3 +236,71 @@ class Context:
 
     def sync(self) -> int:
         return self._lib.p3x_sync(self.handle)
+
+
+def batch_defaults(**overrides) -> BatchOptions:
+    o = BatchOptions()
+    load().p3x_batch_defaults(C.byref(o))
+    for k, v in overrides.items():
+        setattr(o, k, v)
+    return o
+
+
+def column_set(f: dict, nj: int, nk: int, dt: float, predict_nc=True, prescribed_ccn=False,
+               exchange_zeroed=False, ld: int = 0, debug: dict | None = None) -> ColumnSet:
+    """The descriptor of one column set of a batch; ``f`` maps every field name to a device array."""
+    dbg = debug or {}
+    inf = Infrastructure(dt=float(dt), it=1, its=0, ite=nj - 1, kts=0, kte=nk - 1,
+                         predictNc=int(bool(predict_nc)), prescribedCCN=int(bool(prescribed_ccn)),
+                         col_location=None, exchange_zeroed_on_entry=int(bool(exchange_zeroed)))
+    return ColumnSet(PrognosticState(*[_addr(f[k]) for k in STATE]),
+                     DiagnosticInputs(*[_addr(f[k]) for k in INPUTS]),
+                     DiagnosticOutputs(*[_addr(f[k]) for k in OUTS]),
+                     HistoryOnly(*[_addr(f[k]) for k in HIST]), inf, int(nj), int(nk), int(ld),
+                     _addr(dbg.get("col_status")), _addr(dbg.get("sed_substeps")),
+                     _addr(dbg.get("ice_table_idx")))
+
+
+class Batch:
+    """Many column sets advanced one P3 step each per ``run`` (p3x_batch_*)."""
+
+    def __init__(self, ctx: Context, sets: list, options: BatchOptions | None = None,
+                 runtime: Runtime | None = None):
+        self._lib = load()
+        self.ctx = ctx
+        self.sets = (ColumnSet * len(sets))(*sets)
+        rt = runtime if runtime is not None else runtime_defaults()
+        opts = options if options is not None else batch_defaults()
+        h = C.c_void_p()
+        st = self._lib.p3x_batch_create(ctx.handle, C.byref(rt), self.sets, len(sets), C.byref(opts),
+                                        C.byref(h))
+        if st != P3X_OK:
+            raise P3Error(st, "p3x_batch_create")
+        self.handle = h
+        self.options = opts
+
+    def run(self, stream=None):
+        st = self._lib.p3x_batch_run(self.handle, stream)
+        if st != P3X_OK:
+            raise P3Error(st, "p3x_batch_run")
+
+    def tune(self, stream=None, reps: int = 3):
+        """Autotune the launch shape on this batch's data (state left unchanged);
+        returns (chosen options, best device microseconds per step)."""
+        chosen, us = BatchOptions(), C.c_double(0)
+        st = self._lib.p3x_batch_tune(self.handle, stream, int(reps), C.byref(chosen), C.byref(us))
+        if st != P3X_OK:
+            raise P3Error(st, "p3x_batch_tune")
+        self.options = chosen
+        return chosen, us.value
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self._lib.p3x_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -6,8 +6,9 @@ from scream_amd import p3x
 FLOAT_FIELDS = p3x.STATE + p3x.OUTS + p3x.HIST
 
 
-def run_gpu(ctx, inp, dt, predict_nc=True, prescribed_ccn=False, exch=None, **kw):
-    """One p3x_main on device copies of ``inp``; returns (fields, debug, microseconds)."""
+def device_fields(inp, exch=None):
+    """Device copies of ``inp`` plus NaN-filled outputs, zeroed (or ``exch``) exchange
+    fields, and integer debug arrays filled with -7; returns (fields, debug)."""
     import torch
     dev = torch.device("cuda:0")
     nlev, ncol = inp["qc"].shape
@@ -21,11 +22,23 @@ def run_gpu(ctx, inp, dt, predict_nc=True, prescribed_ccn=False, exch=None, **kw
             shape = (nlev, ncol)
         d[k] = torch.full(shape, np.nan, dtype=torch.float64, device=dev)
     for k in p3x.HIST:
-        d[k] = (torch.from_numpy(exch[k]).to(dev) if exch is not None
+        d[k] = (torch.from_numpy(np.ascontiguousarray(exch[k])).to(dev) if exch is not None
                 else torch.zeros((nlev, ncol), dtype=torch.float64, device=dev))
     dbg = dict(col_status=torch.full((ncol,), -7, dtype=torch.int32, device=dev),
                sed_substeps=torch.full((3, ncol), -7, dtype=torch.int32, device=dev),
                ice_table_idx=torch.full((nlev, ncol), -7, dtype=torch.int32, device=dev))
+    return d, dbg
+
+
+def to_host(d, dbg):
+    return ({k: v.cpu().numpy() for k, v in d.items()}, {k: v.cpu().numpy() for k, v in dbg.items()})
+
+
+def run_gpu(ctx, inp, dt, predict_nc=True, prescribed_ccn=False, exch=None, **kw):
+    """One p3x_main on device copies of ``inp``; returns (fields, debug, microseconds)."""
+    import torch
+    nlev, ncol = inp["qc"].shape
+    d, dbg = device_fields(inp, exch)
     torch.cuda.synchronize()
     us = ctx.main(d, ncol, nlev, dt, predict_nc, prescribed_ccn, debug=dbg,
                   stream=torch.cuda.current_stream().cuda_stream, **kw)
