@@ -36,6 +36,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-forms", action="store_true", help="skip the 16384x128 forms")
+    ap.add_argument("--serve-seconds", type=float, default=5.0,
+                    help="length of the served-form run (Poisson arrivals); 0 skips it")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (HBM counter calibration)")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
@@ -115,6 +117,54 @@ def mixed_error(ctx, w, dt):
         out[k] = {"max_rel_to_field_max": (d.max().item() / scale) if scale > 0 else 0.0,
                   "rms_rel": ((d.pow(2).mean().sqrt().item() / rms_ref) if rms_ref > 0 else 0.0)}
     return out
+
+
+def served(ctx, seconds, rate=200.0, members=32, nlev=128, dt=300.0, seed=5):
+    """Served form: Poisson arrivals at ``rate`` requests/s over ``seconds``, each a step of
+    one of ``members`` ensemble members drawn uniformly; member ncol ~ LogNormal(median 2048,
+    sigma 1 in ln-space) clipped to [128, 16384], nlev 128, mixed precision, state resident."""
+    import numpy as np
+    import torch
+    from scream_amd import p3x
+    from scream_amd.inputs import make_columns
+    rng = np.random.default_rng(seed)
+    ncols = np.clip(np.round(2048 * np.exp(rng.normal(0.0, 1.0, members))), 128, 16384).astype(int)
+    srv = p3x.Server(ctx, p3x.server_defaults(precision=p3x.P3X_MIXED))
+    keep = []
+    dev = torch.device("cuda:0")
+    for m, n in enumerate(ncols):
+        inp = make_columns(int(n), nlev, seed=1000 + m)
+        f = {k: torch.from_numpy(inp[k]).to(dev) for k in p3x.STATE + p3x.INPUTS}
+        for k in p3x.OUTS + p3x.HIST:
+            shape = (n,) if "surf" in k else ((nlev + 1, n) if "flux" in k else (nlev, n))
+            f[k] = torch.zeros(shape, dtype=torch.float64, device=dev)
+        keep.append(f)
+        srv.add_member(p3x.column_set(f, int(n), nlev, dt, exchange_zeroed=True))
+    torch.cuda.synchronize()
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, int(rate * seconds * 1.5)))
+    arrivals = arrivals[arrivals < seconds]
+    who = rng.integers(0, members, len(arrivals))
+    t0 = time.perf_counter()
+    for i, (t, m) in enumerate(zip(arrivals, who)):
+        wait = t0 + t - time.perf_counter()
+        if wait > 0:
+            time.sleep(wait)
+        srv.submit(int(m), dt, i)
+    srv.drain()
+    wall = time.perf_counter() - t0
+    bad = [c for c in srv.poll(max_n=len(arrivals) + 1) if c.status != 0]
+    st = srv.stats()
+    srv.close()
+    if bad:
+        raise RuntimeError(f"served form: {len(bad)} requests failed (status {bad[0].status})")
+    return {"requests": int(st.completed), "offered_rate_per_s": rate, "seconds": seconds,
+            "achieved_rate_per_s": st.completed / wall, "p50_ms": st.p50_us / 1000.0,
+            "p99_ms": st.p99_us / 1000.0, "max_ms": st.max_us / 1000.0,
+            "column_levels_per_s": st.column_levels / wall,
+            "members": members, "nlev": nlev, "dt": dt,
+            "member_ncol": {"min": int(ncols.min()), "median": float(np.median(ncols)),
+                            "max": int(ncols.max())},
+            "precision": "mixed", "slots": 4, "max_batch": 32}
 
 
 def cpu_baseline():
@@ -267,6 +317,8 @@ def main():
             "per_field": errs}
         line["forms"] = {f"{ncol2}x{nlev2}_dt{dt2:g}": forms}
         del w2
+    if args.serve_seconds > 0:
+        line["served"] = served(ctx, args.serve_seconds)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline()
     ctx.close()

@@ -144,6 +144,9 @@ typedef struct p3x_launch {
   /* sedimentation + part 3 kernel: lanes per column, 0 for the default, else
    * 16, 32 or 64; nk may not exceed 8 x this (nk <= 512 in any case)         */
   int sed_lanes_per_column;
+  /* part 1 + part 2 kernel: wavefronts per SIMD its registers must allow, 0 for
+   * the default (1), else 1 or 2 (2: 16x256, 32x256, 64x256 only)            */
+  int part12_waves_per_simd;
   void* stream;            /* hipStream_t; NULL = default stream                  */
   int async;               /* 1: return after enqueueing (status from p3x_sync)   */
   /* optional device arrays for the integer diagnostics (NULL = off):
@@ -207,6 +210,7 @@ typedef struct p3x_batch_options {
   int columns_per_block;     /* part 1 + part 2 shape, as in p3x_launch (0 = default)    */
   int threads_per_block;
   int sed_lanes_per_column;  /* 0 = default                                               */
+  int part12_waves_per_simd; /* 0 = default, 1 or 2, as in p3x_launch                    */
   int blocks_per_cu;         /* persistent grid of each launch: this many blocks per CU;  */
                              /* 0 = as many as fit (occupancy); -1 = one block per unit   */
   int use_graph;             /* 1 = capture the launches into a HIP graph once, replay it */
@@ -229,6 +233,88 @@ int p3x_batch_run(p3x_batch* batch, void* stream);
 int p3x_batch_tune(p3x_batch* batch, void* stream, int reps, p3x_batch_options* chosen,
                    double* best_us);
 void p3x_batch_destroy(p3x_batch* batch);
+
+/* ---- the P3Microphysics atmosphere process (eamxx_p3_process_interface.hpp:22-444,
+ * eamxx_p3_run.cpp:5-45): pre-amble, p3_main, post-amble on the field-manager
+ * fields.  Names are the field names the process requests
+ * (eamxx_p3_process_interface.cpp:24-121); every array is a level-major device
+ * array as above, per-column fields have ncol entries. ---- */
+typedef struct p3x_atm_fields {
+  /* required inputs */
+  const double *p_mid, *p_dry_mid, *pseudo_density, *pseudo_density_dry, *cldfrac_tot;
+  const double *nc_nuceat_tend, *nccn, *ni_activated, *inv_qc_relvar;  /* nccn: prescribedCCN */
+  /* updated (wet mixing ratios in and out) */
+  double *T_mid, *qv, *qc, *nc, *qr, *nr, *qi, *qm, *ni, *bm;
+  double *qv_prev_micro_step, *T_prev_micro_step;
+  double *precip_liq_surf_mass, *precip_ice_surf_mass;             /* ncol, accumulated */
+  /* computed */
+  double *eff_radius_qc, *eff_radius_qi, *eff_radius_qr;           /* microns          */
+  double *precip_total_tend, *nevapr, *rainfrac;
+  double *micro_liq_ice_exchange, *micro_vap_liq_exchange, *micro_vap_ice_exchange;
+  /* boundary fluxes of the column conservation checks (ncol each); NULL = off */
+  double *vapor_flux, *water_flux, *ice_flux, *heat_flux;
+} p3x_atm_fields;
+
+typedef struct p3x_process p3x_process;
+
+/* set_grids + initialize_impl: one process for ncol x nlev columns; allocates
+ * the process-local arrays (inv_exner, th_atm, cld_frac_l/i, dz, the
+ * precipitation fluxes, qv2qi_depos_tend, rho_qi). */
+int p3x_process_create(p3x_context* ctx, const p3x_runtime* runtime, int ncol, int nlev,
+                       int predictNc, int prescribedCCN, p3x_process** process);
+/* run_impl(dt): pre-amble, p3_main, post-amble (same stream, same call). */
+int p3x_process_run(p3x_process* process, const p3x_atm_fields* fields, double dt,
+                    const p3x_launch* opts);
+void p3x_process_destroy(p3x_process* process);
+
+/* ---- served form (no reference counterpart): step requests from concurrent
+ * ensemble members, continuously batched.  Members register once; their
+ * arrays (caller-owned device memory) stay resident and are advanced in place.
+ * A scheduler thread admits queued requests into batches as launch slots and
+ * scratch (a fixed HBM budget) free up, one request per member in flight at a
+ * time, in arrival order otherwise; batches on different slots run
+ * concurrently on their own HIP streams.  A request's result equals that of a
+ * p3x_main call on its member alone. ---- */
+typedef struct p3x_server_options {
+  int precision;              /* P3X_FP64 or P3X_MIXED                                */
+  int slots;                  /* batches in flight at once (one stream each); 0 = 4   */
+  int max_batch;              /* requests per batch; 0 = 32                           */
+  double scratch_budget_bytes;/* device scratch of all slots; 0 = 8 GiB               */
+  int columns_per_block, threads_per_block, sed_lanes_per_column;  /* 0 = default  */
+  int part12_waves_per_simd;                                        /* 0 = default  */
+} p3x_server_options;
+
+typedef struct p3x_completion {
+  unsigned long long tag;     /* as given to p3x_server_submit                        */
+  int member;
+  int status;                 /* P3X_OK or the step's error status                    */
+  double latency_us;          /* submit -> completion seen by the scheduler           */
+} p3x_completion;
+
+typedef struct p3x_server_stats {
+  long long submitted, completed;
+  double p50_us, p99_us, max_us;  /* over all completed requests                      */
+  double column_levels;           /* total column-levels stepped                      */
+  double busy_s;                  /* first submit -> last completion                  */
+} p3x_server_stats;
+
+typedef struct p3x_server p3x_server;
+
+void p3x_server_defaults(p3x_server_options* opts);
+int p3x_server_create(p3x_context* ctx, const p3x_runtime* runtime, const p3x_server_options* opts,
+                      p3x_server** server);
+/* Register a member (its column set; infrastructure.dt is replaced per request).
+ * P3X_ERR_ARG if its scratch can never fit one slot's share of the budget. */
+int p3x_server_add_member(p3x_server* server, const p3x_column_set* set, int* member);
+/* Queue one step of member with time step dt; thread-safe. */
+int p3x_server_submit(p3x_server* server, int member, double dt, unsigned long long tag);
+/* Move up to max completed requests into out; waits up to wait_ms for one if
+ * none is ready.  Returns the count (>= 0) or a negative P3X error. */
+int p3x_server_poll(p3x_server* server, p3x_completion* out, int max, int wait_ms);
+/* Wait until every submitted request has completed. */
+int p3x_server_drain(p3x_server* server);
+int p3x_server_stats_get(p3x_server* server, p3x_server_stats* stats);
+void p3x_server_destroy(p3x_server* server);
 
 const char* p3x_strerror(int status);
 

@@ -74,6 +74,19 @@ class _Fields(C.Structure):
     _fields_ = [(k, _dp) for k in STATE + INPUTS + OUTS + HIST]
 
 
+# P3Microphysics fields (p3o_atm in p3_oracle.h), in struct order
+ATM_IN = ("p_mid", "pseudo_density", "pseudo_density_dry", "cldfrac_tot")
+ATM_STATE = ("T_mid", "qv", "qc", "nc", "qr", "nr", "qi", "qm", "ni", "bm", "qv_prev", "T_prev")
+ATM_LOCAL = ("th_atm", "inv_exner", "dz", "cld_frac_l", "cld_frac_i", "cld_frac_r")
+ATM_POST = ("eff_radius_qc", "eff_radius_qi", "eff_radius_qr", "precip_liq_surf",
+            "precip_ice_surf", "precip_liq_surf_mass", "precip_ice_surf_mass",
+            "vapor_flux", "water_flux", "ice_flux", "heat_flux")
+
+
+class _Atm(C.Structure):
+    _fields_ = [(k, _dp) for k in ATM_IN + ATM_STATE + ATM_LOCAL + ATM_POST]
+
+
 class _Debug(C.Structure):
     _fields_ = [("col_status", _ip), ("sed_substeps", _ip), ("ice_table_idx", _ip)]
 
@@ -103,6 +116,8 @@ def load_oracle(math="glibc"):
         lib.p3o_apply_table_ice.argtypes = [C.POINTER(_Tables), C.c_int] + [C.c_double] * 4
         lib.p3o_apply_table_coll.restype = C.c_double
         lib.p3o_apply_table_coll.argtypes = [C.POINTER(_Tables), C.c_int] + [C.c_double] * 6
+        lib.p3o_preamble.argtypes = [C.POINTER(_Atm), C.c_int, C.c_int]
+        lib.p3o_postamble.argtypes = [C.POINTER(_Atm), C.c_int, C.c_int, C.c_double]
         lib.p3o_upwind_step.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_int,
                                         _dp, _dp, _dp, C.POINTER(_dp), C.POINTER(_dp),
                                         C.POINTER(_dp)]
@@ -157,6 +172,70 @@ def run_oracle(inputs: dict, tables: dict, dt: float, predict_nc=True, prescribe
                         float(dt), int(predict_nc), int(prescribed_ccn), int(nthreads))
     del keep
     return f, dbg, nerr
+
+
+def _atm_struct(a):
+    return _Atm(*[(_ptr(a[k]) if a.get(k) is not None else None)
+                  for k in ATM_IN + ATM_STATE + ATM_LOCAL + ATM_POST])
+
+
+def run_process(fields: dict, tables: dict, dt: float, predict_nc=True, prescribed_ccn=False,
+                runtime=None, nthreads=0, math="glibc", fluxes=False):
+    """P3Microphysics::run_impl on copies of the field-manager ``fields``
+    (eamxx_p3_run.cpp:5-45): pre-amble, p3_main, post-amble.  ``fields`` holds
+    p_mid, p_dry_mid, pseudo_density, pseudo_density_dry, cldfrac_tot,
+    nc_nuceat_tend, nccn, ni_activated, inv_qc_relvar, T_mid, qv, qc, nc, qr, nr,
+    qi, qm, ni, bm, qv_prev_micro_step, T_prev_micro_step, precip_liq_surf_mass,
+    precip_ice_surf_mass.  Returns the updated/computed fields by their
+    field-manager names."""
+    lib = load_oracle(math)
+    nlev, ncol = fields["qc"].shape
+    f = {k: np.array(v, dtype=np.float64, order="C") for k, v in fields.items()}
+    a = {k: f[k] for k in ("p_mid", "pseudo_density", "pseudo_density_dry", "cldfrac_tot", "T_mid",
+                           "qv", "qc", "nc", "qr", "nr", "qi", "qm", "ni", "bm")}
+    a["qv_prev"] = f["qv_prev_micro_step"]
+    a["T_prev"] = f["T_prev_micro_step"]
+    for k in ATM_LOCAL:
+        a[k] = np.zeros((nlev, ncol))
+    lib.p3o_preamble(C.byref(_atm_struct(a)), ncol, nlev)
+    # p3_main on the process's views (eamxx_p3_process_interface.cpp:284-337)
+    inp = {"qc": a["qc"], "nc": a["nc"], "qr": a["qr"], "nr": a["nr"], "qi": a["qi"], "qm": a["qm"],
+           "ni": a["ni"], "bm": a["bm"], "qv": a["qv"], "th": a["th_atm"],
+           "nc_nuceat_tend": f["nc_nuceat_tend"], "nccn": f["nccn"], "ni_activated": f["ni_activated"],
+           "inv_qc_relvar": f["inv_qc_relvar"], "cld_frac_i": a["cld_frac_i"],
+           "cld_frac_l": a["cld_frac_l"], "cld_frac_r": a["cld_frac_r"], "pres": f["p_dry_mid"],
+           "dz": a["dz"], "dpres": f["pseudo_density_dry"], "inv_exner": a["inv_exner"],
+           "qv_prev": a["qv_prev"], "t_prev": a["T_prev"]}
+    out, dbg, nerr = run_oracle(inp, tables, dt, predict_nc, prescribed_ccn, runtime, nthreads, math)
+    for k in ("qc", "nc", "qr", "nr", "qi", "qm", "ni", "bm", "qv"):
+        a[k] = out[k]
+    a["th_atm"] = out["th"]
+    a["eff_radius_qc"] = out["diag_eff_radius_qc"]
+    a["eff_radius_qi"] = out["diag_eff_radius_qi"]
+    a["eff_radius_qr"] = out["diag_eff_radius_qr"]
+    a["precip_liq_surf"] = out["precip_liq_surf"]
+    a["precip_ice_surf"] = out["precip_ice_surf"]
+    a["precip_liq_surf_mass"] = f["precip_liq_surf_mass"]
+    a["precip_ice_surf_mass"] = f["precip_ice_surf_mass"]
+    if fluxes:
+        for k in ("vapor_flux", "water_flux", "ice_flux", "heat_flux"):
+            a[k] = np.full(ncol, np.nan)
+    lib.p3o_postamble(C.byref(_atm_struct(a)), ncol, nlev, float(dt))
+    res = {k: a[k] for k in ("T_mid", "qv", "qc", "nc", "qr", "nr", "qi", "qm", "ni", "bm",
+                             "eff_radius_qc", "eff_radius_qi", "eff_radius_qr",
+                             "precip_liq_surf_mass", "precip_ice_surf_mass")}
+    res["qv_prev_micro_step"] = a["qv_prev"]
+    res["T_prev_micro_step"] = a["T_prev"]
+    res["rainfrac"] = a["cld_frac_r"]
+    res["precip_total_tend"] = out["precip_total_tend"]
+    res["nevapr"] = out["nevapr"]
+    res["micro_liq_ice_exchange"] = out["liq_ice_exchange"]
+    res["micro_vap_liq_exchange"] = out["vap_liq_exchange"]
+    res["micro_vap_ice_exchange"] = out["vap_ice_exchange"]
+    if fluxes:
+        for k in ("vapor_flux", "water_flux", "ice_flux", "heat_flux"):
+            res[k] = a[k]
+    return res, dbg, nerr
 
 
 # ---------------------------------------------------------------- Fortran reference

@@ -1524,3 +1524,86 @@ int p3o_main(const p3o_runtime* rt, const p3o_tables* tab, const p3o_fields* f,
   }
   return nerr;
 }
+
+/* ======================================================================
+ * P3Microphysics pre-amble and post-amble
+ * (components/eamxx/src/physics/p3/eamxx_p3_process_interface.hpp:63-194, 200-261),
+ * with the PhysicsFunctions they call
+ * (components/eamxx/src/share/util/scream_common_physics_functions_impl.hpp).
+ * Level-major arrays, k = 0 at the top, element (k, i) at k*ncol + i.
+ * ====================================================================== */
+
+/* exner_function, scream_common_physics_functions_impl.hpp:88-98 */
+static double exner_function(double p) { return pow(p / P0, Rair * INV_CP); }
+
+/* calculate_dz with calculate_virtual_temperature, :214-224, :400-411 */
+static double calculate_dz(double pseudo_density, double p_mid, double T_mid, double qv) {
+  const double c1 = -1.0 + 1.0 / EP_2;
+  const double T_virtual = T_mid * (1.0 + c1 * qv);
+  return (Rair / gravit) * pseudo_density * T_virtual / p_mid;
+}
+
+void p3o_preamble(const p3o_atm* a, int ncol, int nlev) {
+  const double mincld = 0.0001;  /* eamxx_p3_process_interface.hpp:137 */
+  for (int k = 0; k < nlev; ++k) {
+    for (int i = 0; i < ncol; ++i) {
+      const size_t x = (size_t)k * ncol + i;
+      const double pmid = a->p_mid[x], T = a->T_mid[x], cft = a->cldfrac_tot[x];
+      const double dp = a->pseudo_density[x], dp_dry = a->pseudo_density_dry[x];
+      /* :77, dz from the full pressure and the wet qv */
+      a->dz[x] = calculate_dz(dp, pmid, T, a->qv[x]);
+      /* :86-98, calculate_drymmr_from_wetmmr_dp_based (:375-381): wet * dp / dp_dry */
+      double* const q[] = {a->qc, a->nc, a->qr, a->nr, a->qi, a->ni, a->qm, a->bm, a->qv, a->qv_prev};
+      for (int j = 0; j < 10; ++j) q[j][x] = q[j][x] * dp / dp_dry;
+      /* :101-104 */
+      const double exner = exner_function(pmid);
+      a->inv_exner[x] = 1.0 / exner;
+      a->th_atm[x] = T / exner_function(pmid);  /* calculate_theta_from_T, :115-118 */
+      /* :107-109 */
+      a->cld_frac_l[x] = cft > mincld ? cft : mincld;
+      a->cld_frac_i[x] = cft > mincld ? cft : mincld;
+      double cfr = cft > mincld ? cft : mincld;
+      /* :112-124, maximum overlap with the total cloud fraction of the level above */
+      if (k != 0) {
+        const double above = a->cldfrac_tot[x - ncol];
+        cfr = above > cfr ? above : cfr;
+      }
+      a->cld_frac_r[x] = cfr;
+    }
+  }
+}
+
+void p3o_postamble(const p3o_atm* a, int ncol, int nlev, double dt) {
+  for (int k = 0; k < nlev; ++k) {
+    for (int i = 0; i < ncol; ++i) {
+      const size_t x = (size_t)k * ncol + i;
+      const double dp = a->pseudo_density[x], dp_dry = a->pseudo_density_dry[x];
+      /* :210-217, rescaled temperature change; calculate_T_from_theta, :162-165 */
+      const double T_before = a->T_mid[x];
+      double T = (a->th_atm[x] * exner_function(a->p_mid[x]) - T_before) * dp_dry / dp;
+      T += T_before;
+      a->T_mid[x] = T;
+      a->T_prev[x] = T;
+      /* :226-234, calculate_wetmmr_from_drymmr_dp_based (:327-333): dry * dp_dry / dp */
+      double* const q[] = {a->qc, a->nc, a->qr, a->nr, a->qi, a->ni, a->qm, a->bm, a->qv};
+      for (int j = 0; j < 9; ++j) q[j][x] = q[j][x] * dp_dry / dp;
+      a->qv_prev[x] = a->qv[x];
+      /* :238-240 */
+      a->eff_radius_qc[x] *= 1e6;
+      a->eff_radius_qi[x] *= 1e6;
+      a->eff_radius_qr[x] *= 1e6;
+    }
+  }
+  for (int i = 0; i < ncol; ++i) {
+    /* :245-246 */
+    a->precip_liq_surf_mass[i] += a->precip_liq_surf[i] * RHO_H2O * dt;
+    a->precip_ice_surf_mass[i] += a->precip_ice_surf[i] * RHO_H2O * dt;
+    /* :253-258, when the column conservation checks are on */
+    if (a->water_flux) {
+      a->vapor_flux[i] = 0.0;
+      a->water_flux[i] = a->precip_liq_surf[i] + a->precip_ice_surf[i];
+      a->ice_flux[i] = a->precip_ice_surf[i];
+      a->heat_flux[i] = 0.0;
+    }
+  }
+}

@@ -91,6 +91,28 @@ typedef struct p3o_debug {
   int* ice_table_idx;
 } p3o_debug;
 
+/* The P3Microphysics fields its pre-amble and post-amble touch
+ * (components/eamxx/src/physics/p3/eamxx_p3_process_interface.hpp:63-353):
+ * field-manager inputs, the state it converts in place, and the locals it
+ * builds for p3_main.  Level-major; water_flux == NULL turns the optional
+ * column-conservation boundary fluxes off. */
+typedef struct p3o_atm {
+  /* inputs */
+  const double *p_mid, *pseudo_density, *pseudo_density_dry, *cldfrac_tot;
+  /* state: wet <-> dry in place */
+  double *T_mid, *qv, *qc, *nc, *qr, *nr, *qi, *qm, *ni, *bm, *qv_prev, *T_prev;
+  /* pre-amble outputs (p3_main inputs) */
+  double *th_atm, *inv_exner, *dz, *cld_frac_l, *cld_frac_i, *cld_frac_r;
+  /* post-amble: p3_main outputs and the accumulated surface masses */
+  double *eff_radius_qc, *eff_radius_qi, *eff_radius_qr;
+  const double *precip_liq_surf, *precip_ice_surf;
+  double *precip_liq_surf_mass, *precip_ice_surf_mass;
+  double *vapor_flux, *water_flux, *ice_flux, *heat_flux;
+} p3o_atm;
+
+void p3o_preamble(const p3o_atm* a, int ncol, int nlev);
+void p3o_postamble(const p3o_atm* a, int ncol, int nlev, double dt);
+
 /* Defaults of P3Runtime. */
 void p3o_runtime_defaults(p3o_runtime* rt);
 

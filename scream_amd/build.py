@@ -23,7 +23,8 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", f"--offload-arch={A
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           f"-I{os.path.join(HERE, '..', 'include')}"]
 
-SOURCES = ["p3_kernel_fp64.hip", "p3_kernel_mixed.hip", "p3x_api.hip", "p3_tables.cpp"]
+SOURCES = ["p3_kernel_fp64.hip", "p3_kernel_mixed.hip", "p3x_api.hip", "p3x_server.hip",
+           "p3x_process.hip", "p3_tables.cpp"]
 
 
 def _compile(src: str) -> str:

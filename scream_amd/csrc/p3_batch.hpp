@@ -26,8 +26,8 @@ __device__ __forceinline__ int find_set(const int* __restrict__ first, int nsets
   return lo;
 }
 
-template <class R, int CB, int NT>
-__global__ __launch_bounds__(NT) void p3_part12_batch(const KParams* __restrict__ sets,
+template <class R, int CB, int NT, int WPE>
+__global__ __launch_bounds__(NT, WPE) void p3_part12_batch(const KParams* __restrict__ sets,
                                                       const int* __restrict__ first, int nsets) {
   __shared__ int s_i[NT];
   const auto sets4 = (const __attribute__((address_space(4))) KParams*)sets;

@@ -156,6 +156,8 @@ __device__ __forceinline__ void part12_block(const KParams& p, int bid, int* s_i
   int status = (col_ok && (cflags & 3)) ? 1 : 0;
 
   // ================= p3_main_part2 (P3I/p3_main_impl_part2.hpp:113-524)
+  // saturation vapour pressure over liquid at Tmelt (melting and wet growth use qv_sat_dry(Tmelt, p))
+  const double e_melt = mk_svp(Tmelt, false, err);
   int hydro = 0;
   if (col_ok) {
     for (int k = k0; k < k1; ++k) {
@@ -183,7 +185,7 @@ __device__ __forceinline__ void part12_block(const KParams& p, int bid, int* s_i
           s.inv_exner = p.inv_exner[x];
           s.qv_prev = p.qv_prev[x]; s.t_prev = p.t_prev[x];
           s.ni_activated = p.ni_activated[x];
-          if (part2_level<R>(rt, p.tab, dt, p.predictNc, p.prescribedCCN, s, err)) hydro = 1;
+          if (part2_level<R>(rt, p.tab, dt, p.predictNc, p.prescribedCCN, e_melt, s, err)) hydro = 1;
           p.qc[x] = s.qc; p.nc[x] = s.nc; p.qr[x] = s.qr; p.nr[x] = s.nr; p.qi[x] = s.qi;
           p.ni[x] = s.ni; p.qm[x] = s.qm; p.bm[x] = s.bm; p.qv[x] = s.qv; p.th[x] = s.th;
           p.qv2qi_depos_tend[x] = s.qv2qi_depos_tend;
@@ -216,8 +218,9 @@ __device__ __forceinline__ void part12_block(const KParams& p, int bid, int* s_i
   if (err) atomicOr(p.err, err);
 }
 
-template <class R, int CB, int NT>
-__global__ __launch_bounds__(NT) void p3_part12_kernel(const KParams p) {
+// WPE: minimum wavefronts per SIMD the register allocation must allow (1 or 2).
+template <class R, int CB, int NT, int WPE>
+__global__ __launch_bounds__(NT, WPE) void p3_part12_kernel(const KParams p) {
   __shared__ int s_i[NT];
   part12_block<R, CB, NT>(p, blockIdx.x, s_i);
 }

@@ -9,11 +9,13 @@ namespace p3d {
 
 // Shape of one step's two launches.
 //   part 1 + part 2: cb columns per block x nt threads per block
-//     (compiled: 16x128, 16x256, 32x256, 64x256, 32x512, 64x512);
+//     (compiled: 16x128, 16x256, 32x256, 64x256, 32x512, 64x512), register
+//     allocation for wpe = 1 or 2 wavefronts per SIMD (2 only for 16x256,
+//     32x256 and 64x256: fewer registers, with spills);
 //   sedimentation + part 3: lpc lanes per column (16, 32 or 64), each lane
 //     holding L = ceil(nlev / lpc) levels (at most 8), 4 wavefronts per block.
 struct Shape {
-  int cb, nt, lpc;
+  int cb, nt, lpc, wpe;
 };
 
 // Largest levels-per-lane count of the sedimentation kernel.

@@ -26,7 +26,7 @@ struct Lvl {
 // level (the per-level contribution to hydrometeorsPresent, :465-508).
 template <class R>
 __device__ bool part2_level(const Runtime& rt, const Tables& tb, double dt, bool predictNc,
-                            bool prescribedCCN, Lvl& s, unsigned& err) {
+                            bool prescribedCCN, double e_melt, Lvl& s, unsigned& err) {
   const double inv_dt = 1 / dt;
   const double latvap = LatVap, latice = LatIce, inv_cp = INV_CP;
   const double T_atm = s.T_atm, rho = s.rho, inv_rho = s.inv_rho, pres = s.pres;
@@ -153,7 +153,7 @@ __device__ bool part2_level(const Runtime& rt, const Tables& tb, double dt, bool
 
     // ice_melting, P3I/p3_ice_melting_impl.hpp:11-52
     if (qi_incld >= R(QSMALL) && T_atm > Tmelt) {
-      const double qsat0 = qv_sat_dry(Tmelt, pres, false, err);
+      const double qsat0 = ep_2 * e_melt / vmax(pres, 1.e-3);  // qv_sat_dry(Tmelt, pres, liquid)
       const R dqs = R(qsat0 - s.qv);
       qi2qr_melt_tend = (vent * (dT_melt * kap - rhor * R(latvap) * dv * dqs) * R(2) * R(Pi) / R(latice)) * ni_incld;
       qi2qr_melt_tend = vmax(qi2qr_melt_tend, R(0));
@@ -166,7 +166,7 @@ __device__ bool part2_level(const Runtime& rt, const Tables& tb, double dt, bool
       const bool any_if = qi_incld >= R(QSMALL) && (qc_incld + qr_incld) >= R(1.0e-6) && t_is_negative;
       const bool any_if_col = any_if && (qc2qi_collect_tend + qr2qi_collect_tend) >= R(1.0e-10);
       if (any_if) {
-        const double qsat0 = qv_sat_dry(Tmelt, pres, false, err);
+        const double qsat0 = ep_2 * e_melt / vmax(pres, 1.e-3);  // qv_sat_dry(Tmelt, pres, liquid)
         const R dqs = R(qsat0 - s.qv);
         R qc_growth_rate = (vent * R(Pi * 2) * (rhor * R(latvap) * dv * dqs - dT_melt * kap) /
                             (R(latice) + R(CpLiq) * dT_melt)) * ni_incld;

@@ -14,25 +14,10 @@
 #include <vector>
 
 #include "../../include/p3x.h"
-#include "p3_launch.hpp"
+#include "p3x_internal.hpp"
 
-struct p3x_context {
-  int device = 0;
-  int ncu = 0;
-  double* d_ice = nullptr;
-  double* d_coll = nullptr;
-  double* d_vn = nullptr;
-  double* d_vm = nullptr;
-  double* d_revap = nullptr;
-  double* d_scratch = nullptr;
-  size_t scratch_bytes = 0;
-  unsigned* d_err = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  hipStream_t last_stream = nullptr;
-  int pending_timing = 0;
-};
 
-namespace {
+namespace p3x_detail {
 
 int dev_fail(hipError_t e) { return e == hipErrorOutOfMemory ? P3X_ERR_NO_MEMORY : P3X_ERR_DEVICE; }
 
@@ -41,6 +26,7 @@ void default_shape(int precision, int nk, p3d::Shape* sh) {
   (void)precision;
   sh->cb = 32;
   sh->nt = 256;
+  sh->wpe = 1;
   sh->lpc = nk <= 32 * p3d::kMaxLevelsPerLane ? 32 : 64;
 }
 
@@ -104,7 +90,9 @@ size_t scratch_bytes_for(int nk, int ld) {
   return (b + 255) & ~(size_t)255;
 }
 
-}  // namespace
+}  // namespace p3x_detail
+
+using namespace p3x_detail;
 
 extern "C" void p3x_runtime_defaults(p3x_runtime* rt) {
   if (!rt) return;
@@ -228,6 +216,7 @@ extern "C" int p3x_main(p3x_context* c, const p3x_runtime* runtime, const p3x_pr
     sh.nt = o.threads_per_block;
   }
   if (o.sed_lanes_per_column != 0) sh.lpc = o.sed_lanes_per_column;
+  if (o.part12_waves_per_simd != 0) sh.wpe = o.part12_waves_per_simd;
   hipStream_t s = (hipStream_t)o.stream;
   if (nj == 0) return P3X_OK;
   if (!fields_present(ps, di, dout, hist)) return P3X_ERR_ARG;
@@ -266,6 +255,66 @@ extern "C" int p3x_main(p3x_context* c, const p3x_runtime* runtime, const p3x_pr
 
 // ============================================================ batched form
 
+namespace p3x_detail {
+
+int resolve_shape(int precision, int nk_max, int cb, int nt, int lpc, int wpe, p3d::Shape* sh) {
+  default_shape(precision, nk_max, sh);
+  if (cb != 0 || nt != 0) {
+    sh->cb = cb;
+    sh->nt = nt;
+  }
+  if (lpc != 0) sh->lpc = lpc;
+  if (wpe != 0) sh->wpe = wpe;
+  if (sh->lpc != 16 && sh->lpc != 32 && sh->lpc != 64) return P3X_ERR_ARG;
+  if (nk_max > sh->lpc * p3d::kMaxLevelsPerLane) return P3X_ERR_ARG;
+  return P3X_OK;
+}
+
+int layout_batch(const p3d::KParams* sets, int n, const p3d::Shape& sh, Layout* out) {
+  Layout& L = *out;
+  L.first.assign(n + 1, 0);
+  for (int i = 0; i < n; ++i) L.first[i + 1] = L.first[i] + (sets[i].ncol + sh.cb - 1) / sh.cb;
+  const int cols_sed = p3d::kSedWaves * (64 / sh.lpc);
+  std::vector<int> cls_of(n);
+  for (int i = 0; i < n; ++i) {
+    cls_of[i] = p3d::sed_levels_per_lane(sets[i].nlev, sh.lpc);
+    if (cls_of[i] == 0) return P3X_ERR_ARG;
+  }
+  L.sed_order.clear();
+  L.sed_first.clear();
+  L.cls.clear();
+  for (int l = 1; l <= p3d::kMaxLevelsPerLane; ++l) {
+    Layout::Cls c{l, (int)L.sed_order.size(), (int)L.sed_first.size(), 0, 0};
+    for (int i = 0; i < n; ++i) {
+      if (cls_of[i] != l) continue;
+      if (c.nsets == 0) L.sed_first.push_back(0);
+      L.sed_order.push_back(i);
+      c.total += (sets[i].ncol + cols_sed - 1) / cols_sed;
+      L.sed_first.push_back(c.total);
+      c.nsets++;
+    }
+    if (c.nsets > 0) L.cls.push_back(c);
+  }
+  return P3X_OK;
+}
+
+int grid_for(int ncu, int blocks_per_cu, int total, int resident) {
+  const long g = blocks_per_cu < 0 ? total
+                 : (long)ncu * (blocks_per_cu > 0 ? blocks_per_cu : std::max(1, resident));
+  return (int)std::max(1L, std::min<long>(g, std::max(total, 1)));
+}
+
+hipError_t occupancy(int precision, const p3d::Shape& sh, int L, int* r12, int* rsed) {
+  return precision == P3X_FP64 ? p3d::occupancy_p3_fp64(sh, L, r12, rsed)
+                               : p3d::occupancy_p3_mixed(sh, L, r12, rsed);
+}
+
+hipError_t launch_plan(int precision, const p3d::BatchPlan& plan, hipStream_t s) {
+  return precision == P3X_FP64 ? p3d::launch_p3_batch_fp64(plan, s) : p3d::launch_p3_batch_mixed(plan, s);
+}
+
+}  // namespace p3x_detail
+
 struct p3x_batch {
   p3x_context* ctx = nullptr;
   p3x_batch_options opt{};
@@ -295,10 +344,7 @@ void free_plan(p3x_batch* b) {
   b->d_first_sed = nullptr;
 }
 
-hipError_t launch_batch(const p3x_batch* b, hipStream_t s) {
-  return b->opt.precision == P3X_FP64 ? p3d::launch_p3_batch_fp64(b->plan, s)
-                                      : p3d::launch_p3_batch_mixed(b->plan, s);
-}
+hipError_t launch_batch(const p3x_batch* b, hipStream_t s) { return launch_plan(b->opt.precision, b->plan, s); }
 
 // Lay the sets out for b->opt: virtual-block offsets of both launches, the
 // sedimentation classes, persistent grid sizes and (use_graph) the graph.
@@ -311,78 +357,44 @@ int build_plan(p3x_batch* b) {
   int nk_max = 1;
   for (const auto& k : b->sets) nk_max = std::max(nk_max, k.nlev);
   p3d::Shape sh;
-  default_shape(b->opt.precision, nk_max, &sh);
-  if (b->opt.columns_per_block != 0 || b->opt.threads_per_block != 0) {
-    sh.cb = b->opt.columns_per_block;
-    sh.nt = b->opt.threads_per_block;
-  }
-  if (b->opt.sed_lanes_per_column != 0) sh.lpc = b->opt.sed_lanes_per_column;
-  if (sh.lpc != 16 && sh.lpc != 32 && sh.lpc != 64) return P3X_ERR_ARG;
-  b->plan = p3d::BatchPlan{};
-  b->plan.shape = sh;
-
-  // part 1 + part 2: every set, caller's order
-  std::vector<int> first(n + 1, 0);
-  for (int i = 0; i < n; ++i) first[i + 1] = first[i] + (b->sets[i].ncol + sh.cb - 1) / sh.cb;
-  // sedimentation: classes of equal levels-per-lane
-  const int cols_sed = p3d::kSedWaves * (64 / sh.lpc);
-  std::vector<int> cls_of(n);
-  for (int i = 0; i < n; ++i) {
-    cls_of[i] = p3d::sed_levels_per_lane(b->sets[i].nlev, sh.lpc);
-    if (cls_of[i] == 0) return P3X_ERR_ARG;
-  }
+  int st = resolve_shape(b->opt.precision, nk_max, b->opt.columns_per_block, b->opt.threads_per_block,
+                         b->opt.sed_lanes_per_column, b->opt.part12_waves_per_simd, &sh);
+  if (st != P3X_OK) return st;
+  Layout lay;
+  st = layout_batch(b->sets.data(), n, sh, &lay);
+  if (st != P3X_OK) return st;
   std::vector<p3d::KParams> sed_sets;
-  std::vector<int> sed_first;
-  std::vector<std::pair<int, int>> cls_span;  // (offset in sed_sets, offset in sed_first)
-  std::vector<int> cls_L;
-  for (int L = 1; L <= p3d::kMaxLevelsPerLane; ++L) {
-    const size_t s0 = sed_sets.size(), f0 = sed_first.size();
-    int acc = 0;
-    for (int i = 0; i < n; ++i) {
-      if (cls_of[i] != L) continue;
-      if (sed_sets.size() == s0) sed_first.push_back(0);
-      sed_sets.push_back(b->sets[i]);
-      acc += (b->sets[i].ncol + cols_sed - 1) / cols_sed;
-      sed_first.push_back(acc);
-    }
-    if (sed_sets.size() > s0) {
-      cls_span.push_back({(int)s0, (int)f0});
-      cls_L.push_back(L);
-    }
-  }
-  e = hipMalloc((void**)&b->d_first, first.size() * sizeof(int));
-  if (e == hipSuccess) e = hipMemcpy(b->d_first, first.data(), first.size() * sizeof(int), hipMemcpyHostToDevice);
+  for (int i : lay.sed_order) sed_sets.push_back(b->sets[i]);
+
+  e = hipMalloc((void**)&b->d_first, lay.first.size() * sizeof(int));
+  if (e == hipSuccess) e = hipMemcpy(b->d_first, lay.first.data(), lay.first.size() * sizeof(int), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMalloc((void**)&b->d_sets_sed, sed_sets.size() * sizeof(p3d::KParams));
   if (e == hipSuccess)
     e = hipMemcpy(b->d_sets_sed, sed_sets.data(), sed_sets.size() * sizeof(p3d::KParams), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMalloc((void**)&b->d_first_sed, sed_first.size() * sizeof(int));
+  if (e == hipSuccess) e = hipMalloc((void**)&b->d_first_sed, lay.sed_first.size() * sizeof(int));
   if (e == hipSuccess)
-    e = hipMemcpy(b->d_first_sed, sed_first.data(), sed_first.size() * sizeof(int), hipMemcpyHostToDevice);
+    e = hipMemcpy(b->d_first_sed, lay.sed_first.data(), lay.sed_first.size() * sizeof(int), hipMemcpyHostToDevice);
   if (e != hipSuccess) return dev_fail(e);
 
-  auto grid_for = [&](int total, int resident) {
-    long g = b->opt.blocks_per_cu < 0 ? total
-             : (long)c->ncu * (b->opt.blocks_per_cu > 0 ? b->opt.blocks_per_cu : std::max(1, resident));
-    return (int)std::max(1L, std::min<long>(g, total));
-  };
+  b->plan = p3d::BatchPlan{};
+  b->plan.shape = sh;
   b->plan.sets = b->d_sets;
   b->plan.first = b->d_first;
   b->plan.nsets = n;
-  b->plan.nclass = (int)cls_L.size();
+  b->plan.nclass = (int)lay.cls.size();
   for (int ci = 0; ci < b->plan.nclass; ++ci) {
+    const Layout::Cls& lc = lay.cls[ci];
     p3d::SedClass& sc = b->plan.cls[ci];
-    sc.L = cls_L[ci];
-    sc.sets = b->d_sets_sed + cls_span[ci].first;
-    sc.first = b->d_first_sed + cls_span[ci].second;
-    const size_t next_s = ci + 1 < b->plan.nclass ? cls_span[ci + 1].first : sed_sets.size();
-    sc.nsets = (int)(next_s - cls_span[ci].first);
+    sc.L = lc.L;
+    sc.sets = b->d_sets_sed + lc.set_off;
+    sc.first = b->d_first_sed + lc.first_off;
+    sc.nsets = lc.nsets;
     int r12 = 1, rsed = 1;
-    e = b->opt.precision == P3X_FP64 ? p3d::occupancy_p3_fp64(sh, sc.L, &r12, &rsed)
-                                      : p3d::occupancy_p3_mixed(sh, sc.L, &r12, &rsed);
+    e = occupancy(b->opt.precision, sh, sc.L, &r12, &rsed);
     if (e == hipErrorInvalidValue) return P3X_ERR_ARG;
     if (e != hipSuccess) return P3X_ERR_DEVICE;
-    sc.grid = grid_for(sed_first[cls_span[ci].second + sc.nsets], rsed);
-    if (ci == 0) b->plan.grid = grid_for(first[n], r12);
+    sc.grid = grid_for(c->ncu, b->opt.blocks_per_cu, lc.total, rsed);
+    if (ci == 0) b->plan.grid = grid_for(c->ncu, b->opt.blocks_per_cu, lay.first[n], r12);
   }
 
   if (b->opt.use_graph) {
@@ -534,7 +546,8 @@ extern "C" int p3x_batch_tune(p3x_batch* b, void* stream, int reps, p3x_batch_op
   p3x_batch_options best = b->opt;
   double best_t = 0;
   if (e == hipSuccess) trial(&best_t);
-  static const int shapes[][2] = {{16, 128}, {16, 256}, {32, 256}, {64, 256}, {32, 512}, {64, 512}};
+  static const int shapes[][3] = {{16, 128, 1}, {16, 256, 1}, {32, 256, 1}, {64, 256, 1}, {32, 512, 1},
+                                  {64, 512, 1},  {16, 256, 2}, {32, 256, 2}, {64, 256, 2}};
   static const int lanes[] = {16, 32, 64};
   static const int bpcs[] = {0, -1, 1, 2, 4};
   auto consider = [&](const p3x_batch_options& cand) {
@@ -555,6 +568,7 @@ extern "C" int p3x_batch_tune(p3x_batch* b, void* stream, int reps, p3x_batch_op
     p3x_batch_options cand = best;
     cand.columns_per_block = sh[0];
     cand.threads_per_block = sh[1];
+    cand.part12_waves_per_simd = sh[2];
     consider(cand);
   }
   for (int l : lanes) {

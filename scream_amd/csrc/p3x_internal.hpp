@@ -1,0 +1,67 @@
+// p3x_internal.hpp -- pieces of the C ABI implementation shared by its
+// translation units (p3x_api.hip, p3x_server.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "../../include/p3x.h"
+#include "p3_launch.hpp"
+
+// A context: device tables, grown-on-demand scratch, error word, timing events.
+struct p3x_context {
+  int device = 0;
+  int ncu = 0;
+  double* d_ice = nullptr;
+  double* d_coll = nullptr;
+  double* d_vn = nullptr;
+  double* d_vm = nullptr;
+  double* d_revap = nullptr;
+  double* d_scratch = nullptr;
+  size_t scratch_bytes = 0;
+  unsigned* d_err = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipStream_t last_stream = nullptr;
+  int pending_timing = 0;
+};
+
+namespace p3x_detail {
+
+int dev_fail(hipError_t e);
+// Default launch shape per (precision, nlev).
+void default_shape(int precision, int nk, p3d::Shape* sh);
+// KParams of one column set; scratch points at SC_COUNT*nk*ld doubles followed by ld ints.
+void fill_params(const p3x_context* c, const p3x_runtime* runtime, const p3x_prognostic_state* ps,
+                 const p3x_diagnostic_inputs* di, const p3x_diagnostic_outputs* dout,
+                 const p3x_infrastructure* infra, const p3x_history_only* hist, int nj, int nk, int ld,
+                 int* col_status, int* sed_substeps, int* ice_table_idx, double* scratch,
+                 p3d::KParams* out);
+// Every field pointer of a column set is non-null.
+bool fields_present(const p3x_prognostic_state* ps, const p3x_diagnostic_inputs* di,
+                    const p3x_diagnostic_outputs* dout, const p3x_history_only* hist);
+// Bytes of scratch one column set needs, rounded up to 256.
+size_t scratch_bytes_for(int nk, int ld);
+
+// Launch shape for a batch whose deepest set has nk_max levels: the defaults,
+// overridden by the nonzero options.  P3X_ERR_ARG if not compiled in.
+int resolve_shape(int precision, int nk_max, int cb, int nt, int lpc, int wpe, p3d::Shape* sh);
+
+// Host-side layout of a batch of column sets for one shape.
+struct Layout {
+  std::vector<int> first;      // part 1 + part 2 virtual-block offsets, nsets + 1
+  std::vector<int> sed_order;  // set indices grouped by sedimentation class (ascending L)
+  std::vector<int> sed_first;  // per class nsets_c + 1 offsets, classes concatenated
+  struct Cls {
+    int L, set_off, first_off, nsets, total;
+  };
+  std::vector<Cls> cls;
+};
+int layout_batch(const p3d::KParams* sets, int n, const p3d::Shape& sh, Layout* out);
+// Persistent grid for `total` virtual blocks: blocks_per_cu x CUs (0: the
+// resident count from the occupancy API; -1: one block per virtual block).
+int grid_for(int ncu, int blocks_per_cu, int total, int resident);
+hipError_t occupancy(int precision, const p3d::Shape& sh, int L, int* r12, int* rsed);
+hipError_t launch_plan(int precision, const p3d::BatchPlan& plan, hipStream_t s);
+
+}  // namespace p3x_detail

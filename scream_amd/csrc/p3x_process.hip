@@ -1,0 +1,222 @@
+// p3x_process.hip -- the P3Microphysics atmosphere process around p3_main:
+// pre-amble (wet -> dry mixing ratios, dz, exner, theta, cloud fractions with
+// maximum-overlap rain fraction), the two p3_main kernels, and post-amble (the
+// rescaled temperature change, dry -> wet, effective radii in microns, the
+// accumulated surface precipitation masses and the optional boundary fluxes).
+//
+// Follows components/eamxx/src/physics/p3/eamxx_p3_process_interface.hpp:63-261,
+// eamxx_p3_process_interface.cpp:249-340 (which fields feed p3_main) and
+// eamxx_p3_run.cpp:5-45 (the order of run_impl); PhysicsFunctions from
+// components/eamxx/src/share/util/scream_common_physics_functions_impl.hpp.
+// The pre/post-amble arithmetic is FP64 in every precision form; pow uses the
+// same portable function as the kernels, so the process is reproducible
+// against the oracle bit for bit.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+
+#include "p3x_internal.hpp"
+
+using namespace p3x_detail;
+
+namespace {
+
+constexpr double P0 = p3c::P0;
+
+__device__ __forceinline__ double exner_function(double p) {  // :88-98
+  return p3m_pow(p / P0, p3c::Rair * p3c::INV_CP);
+}
+
+struct AmbleParams {
+  p3x_atm_fields f;
+  double *th_atm, *inv_exner, *dz, *cld_frac_l, *cld_frac_i;
+  const double *precip_liq_surf, *precip_ice_surf;
+  int ncol, nlev;
+  double dt;
+};
+
+// One thread per (level, column).
+__global__ __launch_bounds__(256) void p3_preamble_kernel(const AmbleParams a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = blockIdx.y;
+  if (i >= a.ncol) return;
+  const size_t x = (size_t)k * a.ncol + i;
+  const p3x_atm_fields& f = a.f;
+  const double pmid = f.p_mid[x], T = f.T_mid[x], cft = f.cldfrac_tot[x];
+  const double dp = f.pseudo_density[x], dp_dry = f.pseudo_density_dry[x];
+  // calculate_dz with the virtual temperature of the wet qv (:77; :214-224, :400-411)
+  const double c1 = -1.0 + 1.0 / p3c::ep_2;
+  const double T_virtual = T * (1.0 + c1 * f.qv[x]);
+  a.dz[x] = (p3c::Rair / p3c::gravit) * dp * T_virtual / pmid;
+  // wet -> dry, calculate_drymmr_from_wetmmr_dp_based (:86-98; :375-381)
+  double* const q[] = {f.qc, f.nc, f.qr, f.nr, f.qi, f.ni, f.qm, f.bm, f.qv, f.qv_prev_micro_step};
+#pragma unroll
+  for (int j = 0; j < 10; ++j) q[j][x] = q[j][x] * dp / dp_dry;
+  // exner and theta from the full pressure (:101-104; :115-118)
+  const double exner = exner_function(pmid);
+  a.inv_exner[x] = 1.0 / exner;
+  a.th_atm[x] = T / exner_function(pmid);
+  // cloud fractions with the 1e-4 floor and maximum-overlap rain fraction (:107-124)
+  const double mincld = 0.0001;
+  const double cf = cft > mincld ? cft : mincld;
+  a.cld_frac_l[x] = cf;
+  a.cld_frac_i[x] = cf;
+  double cfr = cf;
+  if (k != 0) {
+    const double above = f.cldfrac_tot[x - a.ncol];
+    cfr = above > cfr ? above : cfr;
+  }
+  f.rainfrac[x] = cfr;
+}
+
+__global__ __launch_bounds__(256) void p3_postamble_kernel(const AmbleParams a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = blockIdx.y;
+  if (i >= a.ncol) return;
+  const p3x_atm_fields& f = a.f;
+  if (k < a.nlev) {
+    const size_t x = (size_t)k * a.ncol + i;
+    const double dp = f.pseudo_density[x], dp_dry = f.pseudo_density_dry[x];
+    // rescaled temperature change (:210-217; calculate_T_from_theta :162-165)
+    const double T_before = f.T_mid[x];
+    double T = (a.th_atm[x] * exner_function(f.p_mid[x]) - T_before) * dp_dry / dp;
+    T += T_before;
+    f.T_mid[x] = T;
+    f.T_prev_micro_step[x] = T;
+    // dry -> wet, calculate_wetmmr_from_drymmr_dp_based (:226-234; :327-333)
+    double* const q[] = {f.qc, f.nc, f.qr, f.nr, f.qi, f.ni, f.qm, f.bm, f.qv};
+#pragma unroll
+    for (int j = 0; j < 9; ++j) q[j][x] = q[j][x] * dp_dry / dp;
+    f.qv_prev_micro_step[x] = f.qv[x];
+    // effective radii in microns (:238-240)
+    f.eff_radius_qc[x] *= 1e6;
+    f.eff_radius_qi[x] *= 1e6;
+    f.eff_radius_qr[x] *= 1e6;
+  } else {
+    // the per-column part (:245-258), on an extra row of the grid
+    f.precip_liq_surf_mass[i] += a.precip_liq_surf[i] * p3c::RHO_H2O * a.dt;
+    f.precip_ice_surf_mass[i] += a.precip_ice_surf[i] * p3c::RHO_H2O * a.dt;
+    if (f.water_flux) {
+      f.vapor_flux[i] = 0.0;
+      f.water_flux[i] = a.precip_liq_surf[i] + a.precip_ice_surf[i];
+      f.ice_flux[i] = a.precip_ice_surf[i];
+      f.heat_flux[i] = 0.0;
+    }
+  }
+}
+
+}  // namespace
+
+struct p3x_process {
+  p3x_context* ctx = nullptr;
+  p3x_runtime rt{};
+  int ncol = 0, nlev = 0, predictNc = 1, prescribedCCN = 0, it = 0;
+  double* mem = nullptr;  // every process-local array below
+  double *inv_exner, *th_atm, *cld_frac_l, *cld_frac_i, *dz, *qv2qi_depos_tend, *rho_qi;
+  double *precip_liq_flux, *precip_ice_flux, *precip_liq_surf, *precip_ice_surf, *unused;
+};
+
+extern "C" int p3x_process_create(p3x_context* c, const p3x_runtime* rt, int ncol, int nlev, int predictNc,
+                                  int prescribedCCN, p3x_process** out) {
+  if (!c || !rt || !out || ncol <= 0 || nlev <= 0 || nlev > 64 * p3d::kMaxLevelsPerLane) return P3X_ERR_ARG;
+  *out = nullptr;
+  p3x_process* p = new p3x_process();
+  p->ctx = c;
+  p->rt = *rt;
+  p->ncol = ncol;
+  p->nlev = nlev;
+  p->predictNc = predictNc ? 1 : 0;
+  p->prescribedCCN = prescribedCCN ? 1 : 0;
+  const size_t lev = (size_t)ncol * nlev, iface = (size_t)ncol * (nlev + 1);
+  const size_t total = 8 * lev + 2 * iface + 2 * (size_t)ncol;
+  const hipError_t e = hipMalloc((void**)&p->mem, total * sizeof(double));
+  if (e != hipSuccess) {
+    delete p;
+    return dev_fail(e);
+  }
+  double* m = p->mem;
+  p->inv_exner = m; m += lev;
+  p->th_atm = m; m += lev;
+  p->cld_frac_l = m; m += lev;
+  p->cld_frac_i = m; m += lev;
+  p->dz = m; m += lev;
+  p->qv2qi_depos_tend = m; m += lev;
+  p->rho_qi = m; m += lev;
+  p->unused = m; m += lev;  // nccn when it is not prescribed (never read by p3_main then)
+  p->precip_liq_flux = m; m += iface;
+  p->precip_ice_flux = m; m += iface;
+  p->precip_liq_surf = m; m += ncol;
+  p->precip_ice_surf = m;
+  *out = p;
+  return P3X_OK;
+}
+
+extern "C" int p3x_process_run(p3x_process* p, const p3x_atm_fields* f, double dt, const p3x_launch* opts) {
+  if (!p || !f || !(dt > 0)) return P3X_ERR_ARG;
+  const void* must[] = {f->p_mid, f->p_dry_mid, f->pseudo_density, f->pseudo_density_dry, f->cldfrac_tot,
+                        f->nc_nuceat_tend, f->ni_activated, f->inv_qc_relvar, f->T_mid, f->qv, f->qc, f->nc,
+                        f->qr, f->nr, f->qi, f->qm, f->ni, f->bm, f->qv_prev_micro_step, f->T_prev_micro_step,
+                        f->precip_liq_surf_mass, f->precip_ice_surf_mass, f->eff_radius_qc, f->eff_radius_qi,
+                        f->eff_radius_qr, f->precip_total_tend, f->nevapr, f->rainfrac,
+                        f->micro_liq_ice_exchange, f->micro_vap_liq_exchange, f->micro_vap_ice_exchange};
+  for (const void* q : must)
+    if (!q) return P3X_ERR_ARG;
+  if (p->prescribedCCN && !f->nccn) return P3X_ERR_ARG;
+  const bool some_flux = f->vapor_flux || f->water_flux || f->ice_flux || f->heat_flux;
+  const bool all_flux = f->vapor_flux && f->water_flux && f->ice_flux && f->heat_flux;
+  if (some_flux && !all_flux) return P3X_ERR_ARG;
+  p3x_launch o;
+  if (opts) o = *opts;
+  else p3x_launch_defaults(&o);
+  if (o.ld != 0 && o.ld != p->ncol) return P3X_ERR_ARG;  // the process's arrays are dense
+  hipStream_t s = (hipStream_t)o.stream;
+
+  AmbleParams a;
+  a.f = *f;
+  a.th_atm = p->th_atm;
+  a.inv_exner = p->inv_exner;
+  a.dz = p->dz;
+  a.cld_frac_l = p->cld_frac_l;
+  a.cld_frac_i = p->cld_frac_i;
+  a.precip_liq_surf = p->precip_liq_surf;
+  a.precip_ice_surf = p->precip_ice_surf;
+  a.ncol = p->ncol;
+  a.nlev = p->nlev;
+  a.dt = dt;
+  const dim3 blk(256), grid((p->ncol + 255) / 256, p->nlev);
+  hipLaunchKernelGGL(p3_preamble_kernel, grid, blk, 0, s, a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return P3X_ERR_DEVICE;
+
+  // p3_main on the process's views (eamxx_p3_process_interface.cpp:284-337)
+  p3x_prognostic_state ps = {f->qc, f->nc, f->qr, f->nr, f->qi, f->qm, f->ni, f->bm, f->qv, p->th_atm};
+  p3x_diagnostic_inputs di = {f->nc_nuceat_tend, p->prescribedCCN ? f->nccn : p->unused, f->ni_activated,
+                              f->inv_qc_relvar, p->cld_frac_i, p->cld_frac_l, f->rainfrac, f->p_dry_mid,
+                              p->dz, f->pseudo_density_dry, p->inv_exner, f->qv_prev_micro_step,
+                              f->T_prev_micro_step};
+  p3x_diagnostic_outputs dout = {p->qv2qi_depos_tend, p->precip_liq_surf, p->precip_ice_surf,
+                                 f->eff_radius_qc, f->eff_radius_qi, f->eff_radius_qr, p->rho_qi,
+                                 p->precip_liq_flux, p->precip_ice_flux, f->precip_total_tend, f->nevapr};
+  p3x_history_only hist = {f->micro_liq_ice_exchange, f->micro_vap_liq_exchange, f->micro_vap_ice_exchange};
+  // run_impl zeroes the three exchange fields before p3_main (eamxx_p3_run.cpp:27-29)
+  p3x_infrastructure inf = {dt, ++p->it, 0, p->ncol - 1, 0, p->nlev - 1, p->predictNc, p->prescribedCCN,
+                            nullptr, 1};
+  p3x_launch mo = o;
+  mo.async = 1;
+  int st = p3x_main(p->ctx, &p->rt, &ps, &di, &dout, &inf, &hist, p->ncol, p->nlev, &mo, nullptr);
+  if (st != P3X_OK) return st;
+
+  const dim3 grid2((p->ncol + 255) / 256, p->nlev + 1);
+  hipLaunchKernelGGL(p3_postamble_kernel, grid2, blk, 0, s, a);
+  e = hipGetLastError();
+  if (e != hipSuccess) return P3X_ERR_DEVICE;
+  if (o.async) return P3X_OK;
+  return p3x_sync(p->ctx);
+}
+
+extern "C" void p3x_process_destroy(p3x_process* p) {
+  if (!p) return;
+  (void)hipDeviceSynchronize();
+  (void)hipFree(p->mem);
+  delete p;
+}

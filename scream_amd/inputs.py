@@ -153,3 +153,35 @@ def empty_outputs(ncol: int, nlev: int) -> dict:
     out.update({k: np.zeros((nlev + 1, ncol)) for k in FLUX_FIELDS})
     out.update({k: np.zeros(ncol) for k in SURF_FIELDS})
     return out
+
+
+ATM_IN_FIELDS = ("p_mid", "p_dry_mid", "pseudo_density", "pseudo_density_dry", "cldfrac_tot",
+                 "nc_nuceat_tend", "nccn", "ni_activated", "inv_qc_relvar")
+ATM_UPDATED_FIELDS = ("T_mid", "qv", "qc", "nc", "qr", "nr", "qi", "qm", "ni", "bm",
+                      "qv_prev_micro_step", "T_prev_micro_step", "precip_liq_surf_mass",
+                      "precip_ice_surf_mass")
+ATM_COMPUTED_FIELDS = ("eff_radius_qc", "eff_radius_qi", "eff_radius_qr", "precip_total_tend",
+                       "nevapr", "rainfrac", "micro_liq_ice_exchange", "micro_vap_liq_exchange",
+                       "micro_vap_ice_exchange")
+ATM_FLUX_FIELDS = ("vapor_flux", "water_flux", "ice_flux", "heat_flux")
+
+
+def make_atm_fields(ncol: int, nlev: int, seed: int = 0, **kw) -> dict:
+    """Field-manager fields of the P3Microphysics process (wet mixing ratios) from
+    the same generator as make_columns: its mixing ratios are taken as wet;
+    pseudo_density is its dpres and pseudo_density_dry = pseudo_density (1 - qv);
+    p_mid is its pres and p_dry_mid = p_mid (1 - qv); cldfrac_tot is its cloud
+    fraction before the 1e-4 floor; T_mid = th / inv_exner; the accumulated
+    precipitation masses start at 0."""
+    c = make_columns(ncol, nlev, seed=seed, **kw)
+    T = c["th"] / c["inv_exner"]
+    qv = c["qv"]
+    cf = np.where(c["cld_frac_l"] <= 1e-4, 0.0, c["cld_frac_l"])
+    f = dict(p_mid=c["pres"], p_dry_mid=c["pres"] * (1 - qv), pseudo_density=c["dpres"],
+             pseudo_density_dry=c["dpres"] * (1 - qv), cldfrac_tot=cf,
+             nc_nuceat_tend=c["nc_nuceat_tend"], nccn=c["nccn"], ni_activated=c["ni_activated"],
+             inv_qc_relvar=c["inv_qc_relvar"], T_mid=T, qv=qv, qc=c["qc"], nc=c["nc"], qr=c["qr"],
+             nr=c["nr"], qi=c["qi"], qm=c["qm"], ni=c["ni"], bm=c["bm"],
+             qv_prev_micro_step=qv.copy(), T_prev_micro_step=T.copy(),
+             precip_liq_surf_mass=np.zeros(ncol), precip_ice_surf_mass=np.zeros(ncol))
+    return {k: np.ascontiguousarray(v, dtype=np.float64) for k, v in f.items()}

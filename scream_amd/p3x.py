@@ -38,7 +38,19 @@ ALL_FIELDS = STATE + INPUTS + OUTS + HIST
 # Every symbol include/p3x.h declares.
 EXPORTS = ("p3x_runtime_defaults", "p3x_launch_defaults", "p3x_tables_load", "p3x_init",
            "p3x_finalize", "p3x_main", "p3x_sync", "p3x_strerror", "p3x_batch_defaults",
-           "p3x_batch_create", "p3x_batch_run", "p3x_batch_tune", "p3x_batch_destroy")
+           "p3x_batch_create", "p3x_batch_run", "p3x_batch_tune", "p3x_batch_destroy",
+           "p3x_server_defaults", "p3x_server_create", "p3x_server_add_member", "p3x_server_submit",
+           "p3x_server_poll", "p3x_server_drain", "p3x_server_stats_get", "p3x_server_destroy",
+           "p3x_process_create", "p3x_process_run", "p3x_process_destroy")
+
+# p3x_atm_fields, in struct order (the P3Microphysics field names)
+ATM_FIELDS = ("p_mid", "p_dry_mid", "pseudo_density", "pseudo_density_dry", "cldfrac_tot",
+              "nc_nuceat_tend", "nccn", "ni_activated", "inv_qc_relvar",
+              "T_mid", "qv", "qc", "nc", "qr", "nr", "qi", "qm", "ni", "bm",
+              "qv_prev_micro_step", "T_prev_micro_step", "precip_liq_surf_mass", "precip_ice_surf_mass",
+              "eff_radius_qc", "eff_radius_qi", "eff_radius_qr", "precip_total_tend", "nevapr", "rainfrac",
+              "micro_liq_ice_exchange", "micro_vap_liq_exchange", "micro_vap_ice_exchange",
+              "vapor_flux", "water_flux", "ice_flux", "heat_flux")
 
 
 class Runtime(C.Structure):
@@ -78,7 +90,7 @@ class TableData(C.Structure):
 class Launch(C.Structure):
     _fields_ = [("precision", C.c_int), ("ld", C.c_int), ("columns_per_block", C.c_int),
                 ("threads_per_block", C.c_int), ("sed_lanes_per_column", C.c_int),
-                ("stream", C.c_void_p), ("async_", C.c_int),
+                ("part12_waves_per_simd", C.c_int), ("stream", C.c_void_p), ("async_", C.c_int),
                 ("col_status", C.c_void_p), ("sed_substeps", C.c_void_p),
                 ("ice_table_idx", C.c_void_p)]
 
@@ -92,10 +104,36 @@ class ColumnSet(C.Structure):
 
 class BatchOptions(C.Structure):
     _fields_ = [("precision", C.c_int), ("columns_per_block", C.c_int), ("threads_per_block", C.c_int),
-                ("sed_lanes_per_column", C.c_int), ("blocks_per_cu", C.c_int), ("use_graph", C.c_int)]
+                ("sed_lanes_per_column", C.c_int), ("part12_waves_per_simd", C.c_int),
+                ("blocks_per_cu", C.c_int), ("use_graph", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ServerOptions(C.Structure):
+    _fields_ = [("precision", C.c_int), ("slots", C.c_int), ("max_batch", C.c_int),
+                ("scratch_budget_bytes", C.c_double), ("columns_per_block", C.c_int),
+                ("threads_per_block", C.c_int), ("sed_lanes_per_column", C.c_int),
+                ("part12_waves_per_simd", C.c_int)]
+
+
+class Completion(C.Structure):
+    _fields_ = [("tag", C.c_ulonglong), ("member", C.c_int), ("status", C.c_int),
+                ("latency_us", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_longlong), ("completed", C.c_longlong), ("p50_us", C.c_double),
+                ("p99_us", C.c_double), ("max_us", C.c_double), ("column_levels", C.c_double),
+                ("busy_s", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class AtmFields(C.Structure):
+    _fields_ = [(k, C.c_void_p) for k in ATM_FIELDS]
 
 
 class P3Error(RuntimeError):
@@ -147,6 +185,27 @@ def load() -> C.CDLL:
                                        C.POINTER(C.c_double)]
         lib.p3x_batch_tune.restype = C.c_int
         lib.p3x_batch_destroy.argtypes = [C.c_void_p]
+        lib.p3x_server_defaults.argtypes = [C.POINTER(ServerOptions)]
+        lib.p3x_server_create.argtypes = [C.c_void_p, C.POINTER(Runtime), C.POINTER(ServerOptions),
+                                          C.POINTER(C.c_void_p)]
+        lib.p3x_server_create.restype = C.c_int
+        lib.p3x_server_add_member.argtypes = [C.c_void_p, C.POINTER(ColumnSet), C.POINTER(C.c_int)]
+        lib.p3x_server_add_member.restype = C.c_int
+        lib.p3x_server_submit.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_ulonglong]
+        lib.p3x_server_submit.restype = C.c_int
+        lib.p3x_server_poll.argtypes = [C.c_void_p, C.POINTER(Completion), C.c_int, C.c_int]
+        lib.p3x_server_poll.restype = C.c_int
+        lib.p3x_server_drain.argtypes = [C.c_void_p]
+        lib.p3x_server_drain.restype = C.c_int
+        lib.p3x_server_stats_get.argtypes = [C.c_void_p, C.POINTER(ServerStats)]
+        lib.p3x_server_stats_get.restype = C.c_int
+        lib.p3x_server_destroy.argtypes = [C.c_void_p]
+        lib.p3x_process_create.argtypes = [C.c_void_p, C.POINTER(Runtime), C.c_int, C.c_int, C.c_int,
+                                           C.c_int, C.POINTER(C.c_void_p)]
+        lib.p3x_process_create.restype = C.c_int
+        lib.p3x_process_run.argtypes = [C.c_void_p, C.POINTER(AtmFields), C.c_double, C.POINTER(Launch)]
+        lib.p3x_process_run.restype = C.c_int
+        lib.p3x_process_destroy.argtypes = [C.c_void_p]
         _lib = lib
     return _lib
 
@@ -205,7 +264,7 @@ class Context:
     def main(self, f: dict, nj: int, nk: int, dt: float, predict_nc=True, prescribed_ccn=False,
              runtime: Runtime | None = None, precision: int = P3X_FP64, ld: int = 0,
              columns_per_block: int = 0, threads_per_block: int = 0,
-             sed_lanes_per_column: int = 0, stream=None,
+             sed_lanes_per_column: int = 0, part12_waves_per_simd: int = 0, stream=None,
              async_: bool = False, debug: dict | None = None, exchange_zeroed=False,
              check: bool = True) -> float:
         """One p3x_main call; ``f`` maps every field name to a device array.
@@ -222,7 +281,8 @@ class Context:
         dbg = debug or {}
         lo = Launch(precision=int(precision), ld=int(ld), columns_per_block=int(columns_per_block),
                     threads_per_block=int(threads_per_block),
-                    sed_lanes_per_column=int(sed_lanes_per_column), stream=stream,
+                    sed_lanes_per_column=int(sed_lanes_per_column),
+                    part12_waves_per_simd=int(part12_waves_per_simd), stream=stream,
                     async_=int(bool(async_)), col_status=_addr(dbg.get("col_status")),
                     sed_substeps=_addr(dbg.get("sed_substeps")),
                     ice_table_idx=_addr(dbg.get("ice_table_idx")))
@@ -297,6 +357,117 @@ class Batch:
     def close(self):
         if getattr(self, "handle", None):
             self._lib.p3x_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def server_defaults(**overrides) -> ServerOptions:
+    o = ServerOptions()
+    load().p3x_server_defaults(C.byref(o))
+    for k, v in overrides.items():
+        setattr(o, k, v)
+    return o
+
+
+class Server:
+    """Continuous batching of P3 step requests from ensemble members (p3x_server_*)."""
+
+    def __init__(self, ctx: Context, options: ServerOptions | None = None, runtime: Runtime | None = None):
+        self._lib = load()
+        self.ctx = ctx
+        rt = runtime if runtime is not None else runtime_defaults()
+        opts = options if options is not None else server_defaults()
+        h = C.c_void_p()
+        st = self._lib.p3x_server_create(ctx.handle, C.byref(rt), C.byref(opts), C.byref(h))
+        if st != P3X_OK:
+            raise P3Error(st, "p3x_server_create")
+        self.handle = h
+        self._sets = []  # keep the descriptors alive
+
+    def add_member(self, cs: ColumnSet) -> int:
+        m = C.c_int(-1)
+        self._sets.append(cs)
+        st = self._lib.p3x_server_add_member(self.handle, C.byref(cs), C.byref(m))
+        if st != P3X_OK:
+            raise P3Error(st, "p3x_server_add_member")
+        return m.value
+
+    def submit(self, member: int, dt: float, tag: int = 0):
+        st = self._lib.p3x_server_submit(self.handle, int(member), float(dt), int(tag))
+        if st != P3X_OK:
+            raise P3Error(st, "p3x_server_submit")
+
+    def poll(self, max_n: int = 256, wait_ms: int = 0) -> list:
+        buf = (Completion * max_n)()
+        n = self._lib.p3x_server_poll(self.handle, buf, max_n, int(wait_ms))
+        if n < 0:
+            raise P3Error(-n, "p3x_server_poll")
+        return [buf[i] for i in range(n)]
+
+    def drain(self):
+        st = self._lib.p3x_server_drain(self.handle)
+        if st != P3X_OK:
+            raise P3Error(st, "p3x_server_drain")
+
+    def stats(self) -> ServerStats:
+        s = ServerStats()
+        st = self._lib.p3x_server_stats_get(self.handle, C.byref(s))
+        if st != P3X_OK:
+            raise P3Error(st, "p3x_server_stats_get")
+        return s
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self._lib.p3x_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class P3Microphysics:
+    """The P3 atmosphere process (p3x_process_*): ``run(fields, dt)`` is run_impl --
+    pre-amble, p3_main, post-amble -- on the field-manager fields (device arrays by
+    their field names, see ATM_FIELDS; the four boundary fluxes are optional)."""
+
+    name = "p3"
+
+    def __init__(self, ctx: Context, ncol: int, nlev: int, predict_nc=True, prescribed_ccn=False,
+                 runtime: Runtime | None = None):
+        self._lib = load()
+        self.ctx, self.ncol, self.nlev = ctx, ncol, nlev
+        rt = runtime if runtime is not None else runtime_defaults()
+        h = C.c_void_p()
+        st = self._lib.p3x_process_create(ctx.handle, C.byref(rt), int(ncol), int(nlev),
+                                          int(bool(predict_nc)), int(bool(prescribed_ccn)), C.byref(h))
+        if st != P3X_OK:
+            raise P3Error(st, "p3x_process_create")
+        self.handle = h
+
+    def run(self, fields: dict, dt: float, precision: int = P3X_FP64, stream=None,
+            columns_per_block: int = 0, threads_per_block: int = 0, sed_lanes_per_column: int = 0,
+            async_: bool = False):
+        af = AtmFields(*[_addr(fields.get(k)) for k in ATM_FIELDS])
+        lo = Launch(precision=int(precision), ld=0, columns_per_block=int(columns_per_block),
+                    threads_per_block=int(threads_per_block),
+                    sed_lanes_per_column=int(sed_lanes_per_column),
+                    part12_waves_per_simd=int(part12_waves_per_simd), stream=stream,
+                    async_=int(bool(async_)))
+        st = self._lib.p3x_process_run(self.handle, C.byref(af), float(dt), C.byref(lo))
+        if st != P3X_OK:
+            raise P3Error(st, "p3x_process_run")
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self._lib.p3x_process_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

@@ -93,13 +93,16 @@ def test_fp64_nonzero_history_inputs(ctx, oracle_tables):
     _check_fp64(got, gdbg, want, wdbg, "history in/out")
 
 
-@pytest.mark.parametrize("cb,nt", [(16, 128), (16, 256), (32, 256), (64, 256), (32, 512), (64, 512)])
-def test_fp64_block_shapes_agree(ctx, oracle_tables, cb, nt):
-    """Every compiled block shape gives the same (oracle-matching) answer."""
+@pytest.mark.parametrize("cb,nt,wpe", [(16, 128, 1), (16, 256, 1), (32, 256, 1), (64, 256, 1), (32, 512, 1),
+                                       (64, 512, 1), (16, 256, 2), (32, 256, 2), (64, 256, 2)])
+@pytest.mark.parametrize("lpc", [16, 32, 64])
+def test_fp64_block_shapes_agree(ctx, oracle_tables, cb, nt, wpe, lpc):
+    """Every compiled launch shape gives the same (oracle-matching) answer."""
     inp = make_columns(333, 72, seed=21)
     want, wdbg, _ = oracle.run_oracle(inp, oracle_tables, 1800.0, True, False, math=MATH)
-    got, gdbg, _ = run_gpu(ctx, inp, 1800.0, True, False, columns_per_block=cb, threads_per_block=nt)
-    _check_fp64(got, gdbg, want, wdbg, f"block {cb}x{nt}")
+    got, gdbg, _ = run_gpu(ctx, inp, 1800.0, True, False, columns_per_block=cb, threads_per_block=nt,
+                           part12_waves_per_simd=wpe, sed_lanes_per_column=lpc)
+    _check_fp64(got, gdbg, want, wdbg, f"block {cb}x{nt} waves {wpe} lanes {lpc}")
 
 
 def test_fp64_full_size_g64(ctx, oracle_tables):

@@ -1,0 +1,93 @@
+"""The oracle's P3Microphysics pre-amble and post-amble (CPU).
+
+Pinned by what the mathematics fixes:
+* the pre-amble followed by the post-amble with no microphysics in between
+  returns the wet state (round trip), T unchanged, qv_prev = qv;
+* exner(P0) = 1, so at p = P0 inv_exner = 1 and theta = T;
+* dz with qv = 0 is the dry hydrostatic thickness Rd/g dp T / p;
+* the rain fraction is the maximum-overlap rule, checked by brute force;
+* a whole process step closes the wet column water budget: the water stepped
+  by p3_main on the dry grid is the same water in wet terms;
+* the accumulated surface masses and the boundary fluxes follow from the
+  surface fluxes.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle
+from scream_amd.inputs import make_atm_fields
+
+RD, G, CP, EP_2 = 287.042, 9.80616, 1004.64, 18.016 / 28.966
+
+
+def _atm(f):
+    nlev, ncol = f["qc"].shape
+    a = {k: f[k].copy() for k in ("p_mid", "pseudo_density", "pseudo_density_dry", "cldfrac_tot", "T_mid",
+                                   "qv", "qc", "nc", "qr", "nr", "qi", "qm", "ni", "bm")}
+    a["qv_prev"] = f["qv_prev_micro_step"].copy()
+    a["T_prev"] = f["T_prev_micro_step"].copy()
+    for k in oracle.ATM_LOCAL + ("eff_radius_qc", "eff_radius_qi", "eff_radius_qr"):
+        a[k] = np.zeros((nlev, ncol))
+    for k in ("precip_liq_surf", "precip_ice_surf", "precip_liq_surf_mass", "precip_ice_surf_mass"):
+        a[k] = np.zeros(ncol)
+    return a
+
+
+def test_round_trip_without_microphysics():
+    lib = oracle.load_oracle()
+    f = make_atm_fields(40, 72, seed=3)
+    a = _atm(f)
+    lib.p3o_preamble(C.byref(oracle._atm_struct(a)), 40, 72)
+    lib.p3o_postamble(C.byref(oracle._atm_struct(a)), 40, 72, 300.0)
+    for k in ("qv", "qc", "nc", "qr", "nr", "qi", "qm", "ni", "bm"):
+        np.testing.assert_allclose(a[k], f[k], rtol=4e-16, atol=0, err_msg=k)
+    np.testing.assert_allclose(a["T_mid"], f["T_mid"], rtol=1e-14)
+    np.testing.assert_array_equal(a["T_prev"], a["T_mid"])
+    np.testing.assert_array_equal(a["qv_prev"], a["qv"])
+
+
+def test_exner_reference_pressure_and_dry_dz():
+    lib = oracle.load_oracle()
+    f = make_atm_fields(8, 72, seed=4)
+    f["p_mid"][:] = 1e5
+    f["qv"][:] = 0.0
+    a = _atm(f)
+    lib.p3o_preamble(C.byref(oracle._atm_struct(a)), 8, 72)
+    assert np.all(a["inv_exner"] == 1.0)
+    np.testing.assert_array_equal(a["th_atm"], f["T_mid"])
+    np.testing.assert_allclose(a["dz"], RD / G * f["pseudo_density"] * f["T_mid"] / 1e5, rtol=1e-15)
+
+
+def test_rain_fraction_maximum_overlap_brute_force():
+    lib = oracle.load_oracle()
+    f = make_atm_fields(16, 72, seed=5)
+    a = _atm(f)
+    lib.p3o_preamble(C.byref(oracle._atm_struct(a)), 16, 72)
+    cft = f["cldfrac_tot"]
+    for i in range(16):
+        for k in range(72):
+            want = max(cft[k, i], 1e-4)
+            if k > 0:
+                want = max(want, cft[k - 1, i])
+            assert a["cld_frac_r"][k, i] == want
+            assert a["cld_frac_l"][k, i] == max(cft[k, i], 1e-4) == a["cld_frac_i"][k, i]
+
+
+@pytest.mark.parametrize("dt", [300.0, 1800.0])
+def test_process_step_wet_water_budget(oracle_tables, dt):
+    f = make_atm_fields(64, 72, seed=6, dry_every=8, evap_every=9)
+    out, dbg, nerr = oracle.run_process(f, oracle_tables, dt, fluxes=True)
+    assert nerr == 0
+    wet = lambda s: ((s["qv"] + s["qc"] + s["qr"] + s["qi"]) * f["pseudo_density"] / G).sum(axis=0)
+    before = wet(f)
+    after = wet(out) + out["precip_liq_surf_mass"] + out["precip_ice_surf_mass"]
+    assert np.all(np.abs(after - before) <= 1e-12 * before)
+    # boundary fluxes (eamxx_p3_process_interface.hpp:253-258)
+    np.testing.assert_allclose(out["water_flux"] * 1000.0 * dt,
+                               out["precip_liq_surf_mass"] + out["precip_ice_surf_mass"], rtol=1e-15)
+    assert np.all(out["vapor_flux"] == 0) and np.all(out["heat_flux"] == 0)
+    assert np.all(out["ice_flux"] <= out["water_flux"])
+    # radii in microns: the part-3 defaults 10, 25 and 500 um where nothing is present
+    assert out["eff_radius_qc"].max() >= 10.0 and out["eff_radius_qr"].max() >= 500.0
