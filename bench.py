@@ -36,6 +36,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-forms", action="store_true", help="skip the 16384x128 forms")
+    ap.add_argument("--batch-sets", type=int, default=64,
+                    help="column sets of the batched-form leg; 0 skips it")
     ap.add_argument("--serve-seconds", type=float, default=5.0,
                     help="length of the served-form run (Poisson arrivals); 0 skips it")
     ap.add_argument("--copy-probe", action="store_true",
@@ -119,6 +121,56 @@ def mixed_error(ctx, w, dt):
     return out
 
 
+def batched(ctx, nsets, steps, seed=4):
+    """Batched form: nsets independent column sets, ncol log-uniform in [256, 8192],
+    nlev from {72, 128}, dt from {100, 300, 1800} s, mixed precision, one graph replay
+    per step; launch shape from the autotuner.  The sets' state advances step after
+    step (no restore), as in a run."""
+    import numpy as np
+    import torch
+    from scream_amd import p3x
+    from scream_amd.inputs import make_columns
+    rng = np.random.default_rng(seed)
+    ncols = np.round(np.exp(rng.uniform(np.log(256), np.log(8192), nsets))).astype(int)
+    nlevs = rng.choice([72, 128], nsets)
+    dts = rng.choice([100.0, 300.0, 1800.0], nsets)
+    dev = torch.device("cuda:0")
+    keep, sets = [], []
+    for s in range(nsets):
+        n, k = int(ncols[s]), int(nlevs[s])
+        inp = make_columns(n, k, seed=2000 + s)
+        f = {key: torch.from_numpy(inp[key]).to(dev) for key in p3x.STATE + p3x.INPUTS}
+        for key in p3x.OUTS + p3x.HIST:
+            shape = (n,) if "surf" in key else ((k + 1, n) if "flux" in key else (k, n))
+            f[key] = torch.zeros(shape, dtype=torch.float64, device=dev)
+        keep.append(f)
+        sets.append(p3x.column_set(f, n, k, float(dts[s]), exchange_zeroed=True))
+    torch.cuda.synchronize()
+    b = p3x.Batch(ctx, sets, p3x.batch_defaults(precision=p3x.P3X_MIXED, use_graph=1))
+    sp = torch.cuda.current_stream().cuda_stream
+    chosen, tuned_us = b.tune(sp, reps=2)
+    for _ in range(2):
+        b.run(sp)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record()
+    for _ in range(steps):
+        b.run(sp)
+    e1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    st = ctx.sync()
+    b.close()
+    if st != 0:
+        raise RuntimeError(f"batched form: p3x status {st}")
+    colev = float((ncols * nlevs).sum())
+    return {"sets": nsets, "column_levels_per_step": colev, "steps": steps,
+            "ms_per_step": 1000.0 * wall / steps, "kernel_ms_per_step": e0.elapsed_time(e1) / steps,
+            "column_levels_per_s": colev * steps / wall, "precision": "mixed", "graph": True,
+            "tuned": chosen.as_dict(), "tuned_us_per_step": tuned_us}
+
+
 def served(ctx, seconds, rate=200.0, members=32, nlev=128, dt=300.0, seed=5):
     """Served form: Poisson arrivals at ``rate`` requests/s over ``seconds``, each a step of
     one of ``members`` ensemble members drawn uniformly; member ncol ~ LogNormal(median 2048,
@@ -168,29 +220,56 @@ def served(ctx, seconds, rate=200.0, members=32, nlev=128, dt=300.0, seed=5):
 
 
 def cpu_baseline():
-    """The oracle (FP64 C restatement, OpenMP over columns) on a bounded sample of the
-    headline workload: 8192 columns x 72 levels, dt 1800 s, median of 5 runs."""
+    """The CPU path timed on this host on a bounded sample of the headline workload
+    (8192 columns x 72 levels, dt 1800 s, same generator), median of 5 runs after a
+    discarded cold run.  With oracle/_ref built (the reference's own Fortran P3,
+    micro_p3.F90, compiled from its sources) that is the baseline ("reference"),
+    called on 64-column chunks from a thread pool; the C restatement of the oracle
+    (OpenMP over columns) is timed beside it ("port_value")."""
+    import threading
+    from concurrent.futures import ThreadPoolExecutor
+
+    import numpy as np
+
     import oracle
     from scream_amd import p3x
     from scream_amd.inputs import make_columns
-    import numpy as np
     t = p3x.tables_load()
     a = lambda x, sh: np.ctypeslib.as_array(x).reshape(sh).copy()
     tabs = dict(ice=a(t.ice_table, (5, 4, 50, 12)), coll=a(t.collect_table, (5, 4, 50, 30, 2)),
                 vn=a(t.vn_table, (300, 10)), vm=a(t.vm_table, (300, 10)),
                 revap=a(t.revap_table, (300, 10)), mu_r=a(t.mu_r_table, (150,)), dnu=a(t.dnu, (16,)))
-    ncol, nlev = 8192, 72
+    ncol, nlev, dt = 8192, 72, 1800.0
     inp = make_columns(ncol, nlev, seed=1)
     nthreads = max(1, min(16, os.cpu_count() or 1))
-    times = []
-    for _ in range(5):
-        t0 = time.perf_counter()
-        oracle.run_oracle(inp, tabs, 1800.0, True, False, nthreads=nthreads)
-        times.append(time.perf_counter() - t0)
-    med = statistics.median(times)
-    return {"value": ncol * nlev / med, "unit": "column-levels/s", "cores": nthreads, "kind": "port",
-            "sample": f"oracle/p3_oracle.c FP64, {ncol}x{nlev} columns of the same generator, "
-                      f"dt 1800 s, median of 5 runs ({med:.3f} s each)"}
+
+    def median_time(fn):
+        fn()
+        times = []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            fn()
+            times.append(time.perf_counter() - t0)
+        return statistics.median(times)
+
+    port = median_time(lambda: oracle.run_oracle(inp, tabs, dt, True, False, nthreads=nthreads))
+    out = {"value": ncol * nlev / port, "unit": "column-levels/s", "cores": nthreads, "kind": "port",
+           "sample": f"oracle/p3_oracle.c FP64 (OpenMP), {ncol}x{nlev} columns of the same generator, "
+                     f"dt 1800 s, median of 5 runs ({port:.3f} s each)"}
+    if oracle.ref_available():
+        table_dir = os.path.join(ROOT, "scream_amd", "data")
+        chunks = [{k: np.ascontiguousarray(v[:, c:c + 64]) for k, v in inp.items()} for c in range(0, ncol, 64)]
+        oracle.load_ref(table_dir)  # p3_init (table file I/O) once, before any thread calls p3_main
+        threading.stack_size(256 << 20)  # the Fortran keeps its column arrays on the stack
+        with ThreadPoolExecutor(nthreads) as ex:
+            run = lambda: list(ex.map(lambda ch: oracle.run_ref(ch, table_dir, dt, True, False), chunks))
+            ref = median_time(run)
+        out = {"value": ncol * nlev / ref, "unit": "column-levels/s", "cores": nthreads, "kind": "reference",
+               "sample": f"reference Fortran p3_main (micro_p3.F90 via oracle/_ref), {ncol}x{nlev} columns of "
+                         f"the same generator in 64-column chunks on {nthreads} threads, dt 1800 s, median of "
+                         f"5 runs ({ref:.3f} s each)",
+               "port_value": out["value"], "port_sample": out["sample"]}
+    return out
 
 
 COPY_PROBE_BYTES = 1 << 30
@@ -317,6 +396,8 @@ def main():
             "per_field": errs}
         line["forms"] = {f"{ncol2}x{nlev2}_dt{dt2:g}": forms}
         del w2
+    if args.batch_sets > 0:
+        line["batched"] = batched(ctx, args.batch_sets, max(5, args.steps // 2))
     if args.serve_seconds > 0:
         line["served"] = served(ctx, args.serve_seconds)
     if not args.no_cpu:

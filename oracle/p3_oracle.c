@@ -106,6 +106,20 @@ static const double lookup_table_1a_dum1_c = 4.135985029041767e+00;
 #define P3_EXPM1(x) expm1(x)
 #endif
 
+/* x^bcn.  In the C++ reference bcn is the compile-time constant 2
+ * (physics_constants.hpp), so pow(x, bcn) is the correctly rounded x*x.  In
+ * micro_p3.F90 bcn is a module variable set at run time
+ * (micro_p3_utils.F90:166), so x**bcn is a library pow call, and glibc's
+ * pow(x, 2) differs from x*x in the last bit for about one argument in 1300. */
+#if defined(P3O_FORTRAN_MATH)
+static double pow_bcn(double x) {
+  volatile double y = bcn;
+  return pow(x, y);
+}
+#else
+#define pow_bcn(x) pow((x), bcn)
+#endif
+
 /* ekat::impl::min/max on scalars (ternary form) */
 static inline double dmin(double a, double b) { return a < b ? a : b; }
 static inline double dmax(double a, double b) { return a > b ? a : b; }
@@ -786,7 +800,7 @@ static void p3_column(const p3o_runtime* rt, const p3o_tables* tb, int nk, doubl
         vtrmi1 = table_val_qi_fallspd * w->rhofaci[k];
         if (*qc_incld >= QSMALL) {
           const double mu_c = w->mu_c[k], lamc = w->lamc[k];
-          const double Vt_qc = w->acn[k] * tgamma(4 + bcn + mu_c) / (pow(lamc, bcn) * tgamma(4 + mu_c));
+          const double Vt_qc = w->acn[k] * tgamma(4 + bcn + mu_c) / (pow_bcn(lamc) * tgamma(4 + mu_c));
           const double D_c = (4 + mu_c) / lamc;
           const double V_impact = fabs(vtrmi1 - Vt_qc);
           const double inv_Tc = 1 / dmin(-0.001, T_atm - T_zerodegc);
@@ -1183,7 +1197,7 @@ static void p3_column(const p3o_runtime* rt, const p3o_tables* tb, int nk, doubl
             double nu, cdist, cdist1;
             get_cloud_dsd2(w->qc_incld[k], &w->nc_incld[k], &w->mu_c[k], w->rho[k], &nu, &w->lamc[k], &cdist, &cdist1);
             c->nc[k] = w->nc_incld[k] * c->cld_frac_l[k];
-            const double dum = 1 / pow(w->lamc[k], bcn);
+            const double dum = 1 / pow_bcn(w->lamc[k]);
             w->V1[k] = w->acn[k] * tgamma(4 + bcn + w->mu_c[k]) * dum / tgamma(w->mu_c[k] + 4);
             if (predictNc) w->V2[k] = w->acn[k] * tgamma(1 + bcn + w->mu_c[k]) * dum / tgamma(w->mu_c[k] + 1);
             Co = dmax(0, w->V1[k] * dt_left * w->inv_dz[k]);

@@ -140,4 +140,17 @@ contains
     deallocate(tend_out)
   end subroutine p3ref_main
 
+  ! The elementary functions as micro_p3.F90 evaluates them in this build
+  ! (bfb_math.inc without SCREAM_CONFIG_IS_CMAKE: the Fortran intrinsics), so
+  ! that tests can compare them with the oracle's C library calls.
+  real(c_double) function p3ref_gamma(x) bind(c, name="p3ref_gamma")
+    real(c_double), value, intent(in) :: x
+    p3ref_gamma = gamma(x)
+  end function p3ref_gamma
+
+  real(c_double) function p3ref_pow(x, y) bind(c, name="p3ref_pow")
+    real(c_double), value, intent(in) :: x, y
+    p3ref_pow = x**y
+  end function p3ref_pow
+
 end module p3ref_driver

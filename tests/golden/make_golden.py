@@ -11,6 +11,10 @@ Fixtures (all data, no code):
                        seeded column set (inputs from scream_amd.inputs)
   tables.npz           the rain tables the Fortran p3_init builds, and SHA-256
                        digests of its ice / collection tables
+  harness_mixed_i3_k72_s3.bin  the reference harness's baseline file (format of
+                       components/eamxx/src/physics/p3/tests/p3_run_and_cmp.cpp) written by
+                       the Fortran through scream_amd.harness: mixed initial condition,
+                       3 columns x 72 levels, 3 steps of 300 s, the four flag cases
   lookup.npz           Fortran find_lookupTable_indices_1a/1b +
                        access_lookup_table(_coll) on the literal inputs of
                        components/eamxx/src/physics/p3/tests/p3_ice_tables_unit_tests.cpp:111-155
@@ -104,6 +108,15 @@ def main():
                         rain_in=np.array(LOOKUP_RAIN), ice_idx=ice_idx, ice_w=ice_w,
                         ice_proc=ice_proc, coll_j=coll_j, coll_d3=coll_d3, coll_proc=coll_proc)
     print("wrote lookup")
+
+    # the reference's regression-harness baseline: mixed initial condition, 3 columns,
+    # 72 levels, 3 steps of 300 s, all predict_nc x prescribed_CCN cases, from the Fortran
+    sys.path.insert(0, os.path.dirname(HERE))
+    from harness_runners import FortranRunner  # noqa: E402
+    from scream_amd import harness  # noqa: E402
+    harness.generate_baseline(os.path.join(HERE, "harness_mixed_i3_k72_s3.bin"), FortranRunner(TABLE_DIR),
+                              nsteps=3)
+    print("wrote harness baseline")
 
 
 if __name__ == "__main__":
