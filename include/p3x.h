@@ -267,6 +267,27 @@ int p3x_process_run(p3x_process* process, const p3x_atm_fields* fields, double d
                     const p3x_launch* opts);
 void p3x_process_destroy(p3x_process* process);
 
+/* ---- the mass-and-energy column conservation check that runs after the process
+ * (share/property_checks/mass_and_energy_column_conservation_check.cpp:110-353).
+ * Device arrays, level-major with ld = ncol; levels are summed k = 0 .. nlev-1.
+ * Both calls are enqueued on stream (NULL = default) and return at once. ---- */
+typedef struct p3x_budget_fields {
+  const double *pseudo_density, *T_mid, *u, *v;  /* u, v: horiz_winds; NULL = 0     */
+  const double *qv, *qc, *qi, *qr;
+  const double *ps, *phis;                       /* ncol each; NULL = 0             */
+} p3x_budget_fields;
+
+/* Column total mass [kg/m2] and total energy [J/m2] (:283-301, :312-340). */
+int p3x_column_budget(p3x_context* ctx, const p3x_budget_fields* fields, int ncol, int nlev,
+                      double* mass, double* energy, void* stream);
+/* Per-column relative errors of a step against its boundary fluxes over dt
+ * (:151-154, :171-174, fluxes :303-310, :341-353). */
+int p3x_conservation_errors(p3x_context* ctx, const double* mass_before, const double* energy_before,
+                            const double* mass_after, const double* energy_after,
+                            const double* vapor_flux, const double* water_flux, const double* ice_flux,
+                            const double* heat_flux, int ncol, double dt, double* mass_err,
+                            double* energy_err, void* stream);
+
 /* ---- served form (no reference counterpart): step requests from concurrent
  * ensemble members, continuously batched.  Members register once; their
  * arrays (caller-owned device memory) stay resident and are advanced in place.

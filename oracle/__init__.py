@@ -116,6 +116,8 @@ def load_oracle(math="glibc"):
         lib.p3o_apply_table_ice.argtypes = [C.POINTER(_Tables), C.c_int] + [C.c_double] * 4
         lib.p3o_apply_table_coll.restype = C.c_double
         lib.p3o_apply_table_coll.argtypes = [C.POINTER(_Tables), C.c_int] + [C.c_double] * 6
+        lib.p3o_column_budget.argtypes = [_dp] * 10 + [C.c_int, C.c_int, _dp, _dp]
+        lib.p3o_conservation_errors.argtypes = [_dp] * 8 + [C.c_int, C.c_double, _dp, _dp]
         lib.p3o_preamble.argtypes = [C.POINTER(_Atm), C.c_int, C.c_int]
         lib.p3o_postamble.argtypes = [C.POINTER(_Atm), C.c_int, C.c_int, C.c_double]
         lib.p3o_upwind_step.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_int,
@@ -236,6 +238,28 @@ def run_process(fields: dict, tables: dict, dt: float, predict_nc=True, prescrib
         for k in ("vapor_flux", "water_flux", "ice_flux", "heat_flux"):
             res[k] = a[k]
     return res, dbg, nerr
+
+
+def column_budget(f: dict, math="glibc"):
+    """Column total mass and energy of the conservation check (p3o_column_budget):
+    f holds pseudo_density, T_mid, qv, qc, qi, qr and optionally u, v, ps, phis."""
+    lib = load_oracle(math)
+    nlev, ncol = f["qv"].shape
+    a = {k: (np.ascontiguousarray(f[k], dtype=np.float64) if f.get(k) is not None else None)
+         for k in ("pseudo_density", "T_mid", "u", "v", "qv", "qc", "qi", "qr", "ps", "phis")}
+    mass, energy = np.zeros(ncol), np.zeros(ncol)
+    lib.p3o_column_budget(*[(_ptr(a[k]) if a[k] is not None else None) for k in a], ncol, nlev,
+                          _ptr(mass), _ptr(energy))
+    return mass, energy
+
+
+def conservation_errors(m0, e0, m1, e1, vapor, water, ice, heat, dt, math="glibc"):
+    lib = load_oracle(math)
+    ncol = len(m0)
+    arrs = [np.ascontiguousarray(x, dtype=np.float64) for x in (m0, e0, m1, e1, vapor, water, ice, heat)]
+    me, ee = np.zeros(ncol), np.zeros(ncol)
+    lib.p3o_conservation_errors(*[_ptr(x) for x in arrs], ncol, float(dt), _ptr(me), _ptr(ee))
+    return me, ee
 
 
 # ---------------------------------------------------------------- Fortran reference

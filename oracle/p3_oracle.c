@@ -1621,3 +1621,43 @@ void p3o_postamble(const p3o_atm* a, int ncol, int nlev, double dt) {
     }
   }
 }
+
+/* ======================================================================
+ * Column mass and energy of the reference's conservation property check
+ * (components/eamxx/src/share/property_checks/mass_and_energy_column_conservation_check.cpp:
+ *  compute_total_mass_on_column :283-301, compute_total_energy_on_column :312-340,
+ *  the boundary fluxes :303-310 and :341-353, the relative errors :151-154, :171-174).
+ * Levels are summed in order k = 0 .. nlev-1.  u, v, ps, phis may be NULL (zero).
+ * ====================================================================== */
+void p3o_column_budget(const double* pseudo_density, const double* T_mid, const double* u, const double* v,
+                       const double* qv, const double* qc, const double* qi, const double* qr,
+                       const double* ps, const double* phis, int ncol, int nlev,
+                       double* mass, double* energy) {
+  for (int i = 0; i < ncol; ++i) {
+    double m = 0, e = 0;
+    for (int k = 0; k < nlev; ++k) {
+      const size_t x = (size_t)k * ncol + i;
+      m += (qv[x] + qc[x] + qi[x] + qr[x]) * pseudo_density[x] / gravit;
+      const double u2 = u ? u[x] * u[x] : 0.0;
+      const double v2 = v ? v[x] * v[x] : 0.0;
+      e += (T_mid[x] * Cpair + 0.5 * (u2 + v2) + (LatVap + LatIce) * qv[x] + LatIce * (qc[x] + qr[x])) *
+           pseudo_density[x] / gravit;
+    }
+    e += (phis ? phis[i] : 0.0) * (ps ? ps[i] : 0.0) / gravit;
+    mass[i] = m;
+    energy[i] = e;
+  }
+}
+
+void p3o_conservation_errors(const double* mass0, const double* energy0, const double* mass1,
+                             const double* energy1, const double* vapor_flux, const double* water_flux,
+                             const double* ice_flux, const double* heat_flux, int ncol, double dt,
+                             double* mass_err, double* energy_err) {
+  for (int i = 0; i < ncol; ++i) {
+    const double tm_exp = mass0[i] + (vapor_flux[i] - water_flux[i] * RHO_H2O) * dt;
+    mass_err[i] = fabs(mass1[i] - tm_exp) / mass0[i];
+    const double te_exp = energy0[i] + (vapor_flux[i] * (LatVap + LatIce) -
+                                        (water_flux[i] - ice_flux[i]) * RHO_H2O * LatIce + heat_flux[i]) * dt;
+    energy_err[i] = fabs(energy1[i] - te_exp) / energy0[i];
+  }
+}

@@ -113,6 +113,17 @@ typedef struct p3o_atm {
 void p3o_preamble(const p3o_atm* a, int ncol, int nlev);
 void p3o_postamble(const p3o_atm* a, int ncol, int nlev, double dt);
 
+/* Column total mass and energy, and the relative errors of a step against
+ * the boundary fluxes, of the mass-and-energy column conservation check. */
+void p3o_column_budget(const double* pseudo_density, const double* T_mid, const double* u, const double* v,
+                       const double* qv, const double* qc, const double* qi, const double* qr,
+                       const double* ps, const double* phis, int ncol, int nlev,
+                       double* mass, double* energy);
+void p3o_conservation_errors(const double* mass0, const double* energy0, const double* mass1,
+                             const double* energy1, const double* vapor_flux, const double* water_flux,
+                             const double* ice_flux, const double* heat_flux, int ncol, double dt,
+                             double* mass_err, double* energy_err);
+
 /* Defaults of P3Runtime. */
 void p3o_runtime_defaults(p3o_runtime* rt);
 

@@ -105,7 +105,75 @@ __global__ __launch_bounds__(256) void p3_postamble_kernel(const AmbleParams a) 
   }
 }
 
+struct BudgetParams {
+  p3x_budget_fields f;
+  double *mass, *energy;
+  int ncol, nlev;
+};
+
+// One thread per column, levels in order: the sums round like the oracle's.
+__global__ __launch_bounds__(256) void p3_column_budget_kernel(const BudgetParams b) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= b.ncol) return;
+  const p3x_budget_fields& f = b.f;
+  const double g = p3c::gravit, cp = p3c::CP, lv = p3c::LatVap, lf = p3c::LatIce;
+  double m = 0, e = 0;
+  for (int k = 0; k < b.nlev; ++k) {
+    const size_t x = (size_t)k * b.ncol + i;
+    const double dp = f.pseudo_density[x];
+    const double qv = f.qv[x], qc = f.qc[x], qi = f.qi[x], qr = f.qr[x];
+    m += (qv + qc + qi + qr) * dp / g;
+    const double u2 = f.u ? f.u[x] * f.u[x] : 0.0;
+    const double v2 = f.v ? f.v[x] * f.v[x] : 0.0;
+    e += (f.T_mid[x] * cp + 0.5 * (u2 + v2) + (lv + lf) * qv + lf * (qc + qr)) * dp / g;
+  }
+  e += (f.phis ? f.phis[i] : 0.0) * (f.ps ? f.ps[i] : 0.0) / g;
+  b.mass[i] = m;
+  b.energy[i] = e;
+}
+
+struct ErrParams {
+  const double *m0, *e0, *m1, *e1, *vapor, *water, *ice, *heat;
+  double *merr, *eerr;
+  int ncol;
+  double dt;
+};
+
+__global__ __launch_bounds__(256) void p3_conservation_errors_kernel(const ErrParams a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.ncol) return;
+  const double rho_w = p3c::RHO_H2O, lv = p3c::LatVap, lf = p3c::LatIce;
+  const double tm_exp = a.m0[i] + (a.vapor[i] - a.water[i] * rho_w) * a.dt;
+  a.merr[i] = fabs(a.m1[i] - tm_exp) / a.m0[i];
+  const double te_exp =
+      a.e0[i] + (a.vapor[i] * (lv + lf) - (a.water[i] - a.ice[i]) * rho_w * lf + a.heat[i]) * a.dt;
+  a.eerr[i] = fabs(a.e1[i] - te_exp) / a.e0[i];
+}
+
 }  // namespace
+
+extern "C" int p3x_column_budget(p3x_context* c, const p3x_budget_fields* f, int ncol, int nlev, double* mass,
+                                 double* energy, void* stream) {
+  if (!c || !f || !mass || !energy || ncol < 0 || nlev <= 0) return P3X_ERR_ARG;
+  if (!f->pseudo_density || !f->T_mid || !f->qv || !f->qc || !f->qi || !f->qr) return P3X_ERR_ARG;
+  if (ncol == 0) return P3X_OK;
+  const BudgetParams b{*f, mass, energy, ncol, nlev};
+  hipLaunchKernelGGL(p3_column_budget_kernel, dim3((ncol + 255) / 256), dim3(256), 0, (hipStream_t)stream, b);
+  return hipGetLastError() == hipSuccess ? P3X_OK : P3X_ERR_DEVICE;
+}
+
+extern "C" int p3x_conservation_errors(p3x_context* c, const double* m0, const double* e0, const double* m1,
+                                       const double* e1, const double* vapor_flux, const double* water_flux,
+                                       const double* ice_flux, const double* heat_flux, int ncol, double dt,
+                                       double* mass_err, double* energy_err, void* stream) {
+  if (!c || !m0 || !e0 || !m1 || !e1 || !vapor_flux || !water_flux || !ice_flux || !heat_flux || !mass_err ||
+      !energy_err || ncol < 0)
+    return P3X_ERR_ARG;
+  if (ncol == 0) return P3X_OK;
+  const ErrParams a{m0, e0, m1, e1, vapor_flux, water_flux, ice_flux, heat_flux, mass_err, energy_err, ncol, dt};
+  hipLaunchKernelGGL(p3_conservation_errors_kernel, dim3((ncol + 255) / 256), dim3(256), 0, (hipStream_t)stream, a);
+  return hipGetLastError() == hipSuccess ? P3X_OK : P3X_ERR_DEVICE;
+}
 
 struct p3x_process {
   p3x_context* ctx = nullptr;

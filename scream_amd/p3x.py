@@ -41,7 +41,8 @@ EXPORTS = ("p3x_runtime_defaults", "p3x_launch_defaults", "p3x_tables_load", "p3
            "p3x_batch_create", "p3x_batch_run", "p3x_batch_tune", "p3x_batch_destroy",
            "p3x_server_defaults", "p3x_server_create", "p3x_server_add_member", "p3x_server_submit",
            "p3x_server_poll", "p3x_server_drain", "p3x_server_stats_get", "p3x_server_destroy",
-           "p3x_process_create", "p3x_process_run", "p3x_process_destroy")
+           "p3x_process_create", "p3x_process_run", "p3x_process_destroy", "p3x_column_budget",
+           "p3x_conservation_errors")
 
 # p3x_atm_fields, in struct order (the P3Microphysics field names)
 ATM_FIELDS = ("p_mid", "p_dry_mid", "pseudo_density", "pseudo_density_dry", "cldfrac_tot",
@@ -132,6 +133,13 @@ class ServerStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+BUDGET_FIELDS = ("pseudo_density", "T_mid", "u", "v", "qv", "qc", "qi", "qr", "ps", "phis")
+
+
+class BudgetFields(C.Structure):
+    _fields_ = [(k, C.c_void_p) for k in BUDGET_FIELDS]
+
+
 class AtmFields(C.Structure):
     _fields_ = [(k, C.c_void_p) for k in ATM_FIELDS]
 
@@ -206,6 +214,13 @@ def load() -> C.CDLL:
         lib.p3x_process_run.argtypes = [C.c_void_p, C.POINTER(AtmFields), C.c_double, C.POINTER(Launch)]
         lib.p3x_process_run.restype = C.c_int
         lib.p3x_process_destroy.argtypes = [C.c_void_p]
+        lib.p3x_column_budget.argtypes = [C.c_void_p, C.POINTER(BudgetFields), C.c_int, C.c_int, C.c_void_p,
+                                          C.c_void_p, C.c_void_p]
+        lib.p3x_column_budget.restype = C.c_int
+        lib.p3x_conservation_errors.argtypes = [C.c_void_p] + [C.c_void_p] * 8 + [C.c_int, C.c_double,
+                                                                                 C.c_void_p, C.c_void_p,
+                                                                                 C.c_void_p]
+        lib.p3x_conservation_errors.restype = C.c_int
         _lib = lib
     return _lib
 
@@ -475,3 +490,22 @@ class P3Microphysics:
             self.close()
         except Exception:
             pass
+
+
+def column_budget(ctx: Context, fields: dict, ncol: int, nlev: int, mass, energy, stream=None):
+    """Enqueue p3x_column_budget; fields maps BUDGET_FIELDS names to device arrays
+    (u, v, ps, phis optional)."""
+    bf = BudgetFields(*[_addr(fields.get(k)) for k in BUDGET_FIELDS])
+    st = load().p3x_column_budget(ctx.handle, C.byref(bf), int(ncol), int(nlev), _addr(mass), _addr(energy),
+                                  stream)
+    if st != P3X_OK:
+        raise P3Error(st, "p3x_column_budget")
+
+
+def conservation_errors(ctx: Context, m0, e0, m1, e1, vapor, water, ice, heat, ncol: int, dt: float,
+                        mass_err, energy_err, stream=None):
+    st = load().p3x_conservation_errors(ctx.handle, *[_addr(x) for x in (m0, e0, m1, e1, vapor, water, ice,
+                                                                         heat)],
+                                        int(ncol), float(dt), _addr(mass_err), _addr(energy_err), stream)
+    if st != P3X_OK:
+        raise P3Error(st, "p3x_conservation_errors")

@@ -45,3 +45,37 @@ def test_process_matches_oracle(ctx, oracle_tables, ncol, nlev, pnc, pccn, fluxe
             np.testing.assert_array_equal(d[k].cpu().numpy(), res[k], err_msg=f"step {step} {k}")
         want = {**want, **{k: res[k] for k in ATM_UPDATED_FIELDS}}
     proc.close()
+
+
+def test_conservation_check_matches_oracle_and_closes(ctx, oracle_tables):
+    """p3x_column_budget / p3x_conservation_errors reproduce the oracle bit for bit,
+    and a GPU process step passes the check to roundoff."""
+    import torch
+    dev = torch.device("cuda:0")
+    ncol, nlev, dt = 400, 72, 1800.0
+    f = make_atm_fields(ncol, nlev, seed=91, dry_every=7, evap_every=9)
+    d = {k: torch.from_numpy(v).to(dev) for k, v in f.items()}
+    for k in ATM_COMPUTED_FIELDS:
+        d[k] = torch.zeros((nlev, ncol), dtype=torch.float64, device=dev)
+    for k in ATM_FLUX_FIELDS:
+        d[k] = torch.zeros((ncol,), dtype=torch.float64, device=dev)
+    vec = lambda: torch.zeros(ncol, dtype=torch.float64, device=dev)
+    m0, e0, m1, e1, me, ee = vec(), vec(), vec(), vec(), vec(), vec()
+    sp = torch.cuda.current_stream().cuda_stream
+    p3x.column_budget(ctx, d, ncol, nlev, m0, e0, sp)
+    proc = p3x.P3Microphysics(ctx, ncol, nlev)
+    proc.run(d, dt, stream=sp)
+    p3x.column_budget(ctx, d, ncol, nlev, m1, e1, sp)
+    p3x.conservation_errors(ctx, m0, e0, m1, e1, d["vapor_flux"], d["water_flux"], d["ice_flux"],
+                            d["heat_flux"], ncol, dt, me, ee, sp)
+    torch.cuda.synchronize()
+    proc.close()
+    wm0, we0 = oracle.column_budget(f)
+    np.testing.assert_array_equal(m0.cpu().numpy(), wm0)
+    np.testing.assert_array_equal(e0.cpu().numpy(), we0)
+    after = {k: d[k].cpu().numpy() for k in ("T_mid", "qv", "qc", "qi", "qr")}
+    after["pseudo_density"] = f["pseudo_density"]
+    wm1, we1 = oracle.column_budget(after)
+    np.testing.assert_array_equal(m1.cpu().numpy(), wm1)
+    np.testing.assert_array_equal(e1.cpu().numpy(), we1)
+    assert me.max().item() < 1e-14 and ee.max().item() < 1e-14

@@ -91,3 +91,40 @@ def test_process_step_wet_water_budget(oracle_tables, dt):
     assert np.all(out["ice_flux"] <= out["water_flux"])
     # radii in microns: the part-3 defaults 10, 25 and 500 um where nothing is present
     assert out["eff_radius_qc"].max() >= 10.0 and out["eff_radius_qr"].max() >= 500.0
+
+
+@pytest.mark.parametrize("dt", [100.0, 300.0, 1800.0])
+def test_process_step_passes_column_conservation_check(oracle_tables, dt):
+    """The reference's mass-and-energy column conservation check, run around a
+    process step with P3's boundary fluxes: both close to roundoff.  A 1e-9
+    relative change of the post-step temperature (a plausible slip in the
+    post-amble's rescaled dT) is caught."""
+    f = make_atm_fields(64, 72, seed=7, dry_every=8, evap_every=9)
+    m0, e0 = oracle.column_budget(f)
+    out, _, nerr = oracle.run_process(f, oracle_tables, dt, fluxes=True)
+    assert nerr == 0
+    after = dict(out, pseudo_density=f["pseudo_density"])
+    m1, e1 = oracle.column_budget(after)
+    fl = [out[k] for k in ("vapor_flux", "water_flux", "ice_flux", "heat_flux")]
+    me, ee = oracle.conservation_errors(m0, e0, m1, e1, *fl, dt)
+    assert me.max() < 1e-14 and ee.max() < 1e-14, (me.max(), ee.max())
+    bad = dict(after, T_mid=after["T_mid"] * (1 + 1e-9))
+    _, e_bad = oracle.column_budget(bad)
+    _, ee_bad = oracle.conservation_errors(m0, e0, m1, e_bad, *fl, dt)
+    assert ee_bad.max() > 1e-10
+
+
+def test_column_budget_closed_forms():
+    """Mass of a column of uniform mixing ratios is q * sum(dp) / g; the energy of
+    an isothermal dry calm column is cp T sum(dp) / g plus phis ps / g."""
+    nlev, ncol = 72, 5
+    dp = np.full((nlev, ncol), 1000.0)
+    z = np.zeros((nlev, ncol))
+    q = np.full((nlev, ncol), 1e-3)
+    f = dict(pseudo_density=dp, T_mid=np.full((nlev, ncol), 250.0), qv=q, qc=z, qi=z, qr=z,
+             ps=np.full(ncol, 1e5), phis=np.full(ncol, 100.0))
+    m, e = oracle.column_budget(f)
+    np.testing.assert_allclose(m, 1e-3 * 72000.0 / 9.80616, rtol=1e-14)
+    f["qv"] = z
+    m, e = oracle.column_budget(f)
+    np.testing.assert_allclose(e, (1004.64 * 250.0 * 72000.0 + 100.0 * 1e5) / 9.80616, rtol=1e-14)
