@@ -220,9 +220,8 @@ def served(ctx, seconds, rate=200.0, members=32, nlev=128, dt=300.0, seed=5):
 
 
 def cpu_baseline():
-    """The CPU path timed on this host on a bounded sample of the headline workload
-    (8192 columns x 72 levels, dt 1800 s, same generator), median of 5 runs after a
-    discarded cold run.  With oracle/_ref built (the reference's own Fortran P3,
+    """The CPU path timed on this host on the headline workload itself (21600 columns x
+    72 levels, dt 1800 s, same generator), median of 5 runs after a discarded cold run.  With oracle/_ref built (the reference's own Fortran P3,
     micro_p3.F90, compiled from its sources) that is the baseline ("reference"),
     called on 64-column chunks from a thread pool; the C restatement of the oracle
     (OpenMP over columns) is timed beside it ("port_value")."""
@@ -239,7 +238,7 @@ def cpu_baseline():
     tabs = dict(ice=a(t.ice_table, (5, 4, 50, 12)), coll=a(t.collect_table, (5, 4, 50, 30, 2)),
                 vn=a(t.vn_table, (300, 10)), vm=a(t.vm_table, (300, 10)),
                 revap=a(t.revap_table, (300, 10)), mu_r=a(t.mu_r_table, (150,)), dnu=a(t.dnu, (16,)))
-    ncol, nlev, dt = 8192, 72, 1800.0
+    ncol, nlev, dt = 21600, 72, 1800.0
     inp = make_columns(ncol, nlev, seed=1)
     nthreads = max(1, min(16, os.cpu_count() or 1))
 
@@ -276,24 +275,26 @@ COPY_PROBE_BYTES = 1 << 30
 
 
 def copy_probe():
-    """A plain device-to-device copy of 1 GiB, three times: under rocprofv3 --pmc its
-    FETCH_SIZE / WRITE_SIZE per dispatch calibrate the counters (1 GiB read, 1 GiB written)."""
+    """A plain streaming copy of 1 GiB as one elementwise kernel (dst = src * 1), three
+    times: under rocprofv3 --pmc its FETCH_SIZE / WRITE_SIZE per dispatch calibrate the
+    counters (1 GiB read, 1 GiB written)."""
     import torch
     n = COPY_PROBE_BYTES // 8
     src = torch.ones(n, dtype=torch.float64, device="cuda:0")
     dst = torch.empty_like(src)
     for _ in range(3):
-        dst.copy_(src)
+        torch.mul(src, 1.0, out=dst)
     torch.cuda.synchronize()
     print(json.dumps({"copy_probe_bytes": COPY_PROBE_BYTES}))
 
 
-def _pmc_means(directory, counter):
-    """Mean counter value per dispatch, by kernel name, from rocprofv3 csv output."""
+def _pmc_means(directory, counter, prefix="*"):
+    """Mean counter value per dispatch, by kernel name, from the rocprofv3 csv output
+    files named <prefix>_counter_collection.csv under directory."""
     import csv
     import glob
     sums = {}
-    for path in glob.glob(os.path.join(directory, "**", "*counter_collection.csv"), recursive=True):
+    for path in glob.glob(os.path.join(directory, "**", f"{prefix}_counter_collection.csv"), recursive=True):
         with open(path) as fh:
             for row in csv.DictReader(fh):
                 if row.get("Counter_Name") != counter:
@@ -308,10 +309,10 @@ def _pmc_means(directory, counter):
 def summarize_pmc(fetch_dir, write_dir):
     """HBM bytes per step of the headline workload from FETCH_SIZE and WRITE_SIZE (KiB per
     dispatch), each scaled by the bytes-per-KiB the same counter reports for the copy probe."""
-    fetch, write = _pmc_means(fetch_dir, "FETCH_SIZE"), _pmc_means(write_dir, "WRITE_SIZE")
+    fetch, write = _pmc_means(fetch_dir, "FETCH_SIZE", "bench"), _pmc_means(write_dir, "WRITE_SIZE", "bench")
     pick = lambda d, key: {k: v for k, v in d.items() if key in k}
-    copy_f = max(pick(fetch, "copy").values() or pick(fetch, "elementwise").values())
-    copy_w = max(pick(write, "copy").values() or pick(write, "elementwise").values())
+    copy_f = max(_pmc_means(fetch_dir, "FETCH_SIZE", "copy").values())
+    copy_w = max(_pmc_means(write_dir, "WRITE_SIZE", "copy").values())
     scale_f = COPY_PROBE_BYTES / copy_f  # bytes per counter KiB, reads
     scale_w = COPY_PROBE_BYTES / copy_w  # bytes per counter KiB, writes
     kern = {}
@@ -322,7 +323,8 @@ def summarize_pmc(fetch_dir, write_dir):
             kern[name] = {"FETCH_SIZE_KiB": max(f.values()), "WRITE_SIZE_KiB": max(w.values()),
                           "bytes": max(f.values()) * scale_f + max(w.values()) * scale_w}
     out = {"g64_fp64": {
-        "workload": "p3_main FP64 drop-in, 21600x72, dt=1800 s (bench.py --no-cpu --no-forms)",
+        "workload": "p3_main FP64 drop-in, 21600x72, dt=1800 s (bench.py --no-cpu --no-forms "
+                    "--batch-sets 0 --serve-seconds 0)",
         "copy_probe": {"bytes_each_way": COPY_PROBE_BYTES, "FETCH_SIZE_KiB": copy_f,
                        "WRITE_SIZE_KiB": copy_w, "read_bytes_per_KiB": scale_f,
                        "write_bytes_per_KiB": scale_w},

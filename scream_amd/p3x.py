@@ -469,7 +469,7 @@ class P3Microphysics:
 
     def run(self, fields: dict, dt: float, precision: int = P3X_FP64, stream=None,
             columns_per_block: int = 0, threads_per_block: int = 0, sed_lanes_per_column: int = 0,
-            async_: bool = False):
+            part12_waves_per_simd: int = 0, async_: bool = False):
         af = AtmFields(*[_addr(fields.get(k)) for k in ATM_FIELDS])
         lo = Launch(precision=int(precision), ld=0, columns_per_block=int(columns_per_block),
                     threads_per_block=int(threads_per_block),
