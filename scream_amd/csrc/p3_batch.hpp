@@ -39,7 +39,7 @@ __global__ __launch_bounds__(NT, WPE) void p3_part12_batch(const KParams* __rest
 }
 
 template <class R, int LPC, int L, int WPB>
-__global__ __launch_bounds__(64 * WPB) void p3_sed_batch(const KParams* __restrict__ sets,
+__global__ __launch_bounds__(64 * WPB, sed_waves_per_simd<L>()) void p3_sed_batch(const KParams* __restrict__ sets,
                                                          const int* __restrict__ first, int nsets) {
   const auto sets4 = (const __attribute__((address_space(4))) KParams*)sets;
   const int total = first[nsets];

@@ -91,16 +91,22 @@ __device__ bool part2_level(const Runtime& rt, const Tables& tb, double dt, bool
       s.bm = double(bm_incld) * cld_frac_i;
       const TabIce<R> ti = lookup_ice<R>(qi_incld, ni_incld, qm_incld, rhop);
       const TabRain<R> tr = lookup_rain<R>(qr_incld, nr_incld);
-      tv_qi_fallspd = apply_table_ice<R>(tb.ice, 1, ti);
-      tv_ni_self_collect = apply_table_ice<R>(tb.ice, 2, ti);
-      tv_qc2qi_collect = apply_table_ice<R>(tb.ice, 3, ti);
-      tv_qi2qr_melting = apply_table_ice<R>(tb.ice, 4, ti);
-      tv_lammax = apply_table_ice<R>(tb.ice, 6, ti);
-      tv_lammin = apply_table_ice<R>(tb.ice, 7, ti);
-      tv_vent_melt = apply_table_ice<R>(tb.ice, 9, ti);
+      // ice quantities 1,2,3,4 then 6,7,9 of the part 2 records
+      R v[4];
+      apply_table_ice_rec<4, 8, R>(TabOf<R>::get(tb).ice_p2, ti, v, 0);
+      tv_qi_fallspd = v[0];
+      tv_ni_self_collect = v[1];
+      tv_qc2qi_collect = v[2];
+      tv_qi2qr_melting = v[3];
+      apply_table_ice_rec<4, 8, R>(TabOf<R>::get(tb).ice_p2, ti, v, 4);
+      tv_lammax = v[0];
+      tv_lammin = v[1];
+      tv_vent_melt = v[2];
       if (qr_incld >= R(QSMALL)) {
-        tv_nr_collect = apply_table_coll<R>(tb.coll, 0, ti, tr);
-        tv_qr2qi_collect = apply_table_coll<R>(tb.coll, 1, ti, tr);
+        R c[2];
+        apply_table_coll2<R>(TabOf<R>::get(tb).coll, ti, tr, c);
+        tv_nr_collect = c[0];
+        tv_qr2qi_collect = c[1];
       }
       ni_incld = vmin(ni_incld, tv_lammax * ni_incld);
       ni_incld = vmax(ni_incld, tv_lammin * ni_incld);
@@ -237,7 +243,7 @@ __device__ bool part2_level(const Runtime& rt, const Tables& tb, double dt, bool
     if (qr_incld >= R(QSMALL)) {
       const Tab3<R> t3 = lookup3<R>(mu_r, lamr);
       epsr = R(2) * R(Pi) * cdistr * rhor * dv *
-             (R(f1r) * m_tgamma(mu_r + R(2)) / lamr + R(f2r) * m_sqrt(rhor / mu) * m_cbrt(sc) * apply_table3<R>(tb.revap, t3));
+             (R(f1r) * m_tgamma(mu_r + R(2)) / lamr + R(f2r) * m_sqrt(rhor / mu) * m_cbrt(sc) * apply_table3<R>(TabOf<R>::get(tb).revap, t3));
     }
 
     // evaporate_rain, P3I/p3_evaporate_rain_impl.hpp:83-201

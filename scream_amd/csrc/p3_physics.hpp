@@ -84,12 +84,35 @@ struct Runtime {  // P3Runtime, components/eamxx/src/physics/p3/p3_functions.hpp
 };
 
 // ---------------------------------------------------------------- tables
+// Device copies of the lookup tables, per arithmetic type.  Besides the
+// reference layouts (coll, revap) the ice and rain-velocity tables are
+// repacked into records holding just the quantities one kernel interpolates,
+// so that a table corner is one or two 16-byte loads instead of one load per
+// quantity.  The values and the interpolation arithmetic are those of the
+// reference tables; only the addresses differ.
+template <class T> struct TableSet {
+  const T* coll;     // [5][4][50][30][2]  (log10 values), reference layout
+  const T* revap;    // [300][10]
+  const T* ice_p2;   // [5][4][50][8]: ice quantities 1,2,3,4,6,7,9 and a pad (part 2)
+  const T* ice_sed;  // [5][4][50][4]: quantities 0,1,6,7 (ice sedimentation)
+  const T* ice_p3;   // [5][4][50][4]: quantities 5,6,7,11 (part 3)
+  const T* rain_v;   // [300][10][2]: vm, vn (rain sedimentation)
+};
 struct Tables {
-  const double* ice;      // [5][4][50][12]
-  const double* coll;     // [5][4][50][30][2]  (log10 values)
-  const double* vn;       // [300][10]
-  const double* vm;       // [300][10]
-  const double* revap;    // [300][10]
+  TableSet<double> d;  // the drop-in FP64 form
+  TableSet<float> f;   // the mixed-precision form (half the gather bytes, no conversions)
+};
+constexpr int kIceP2Quant[7] = {1, 2, 3, 4, 6, 7, 9};
+constexpr int kIceSedQuant[4] = {0, 1, 6, 7};
+constexpr int kIceP3Quant[4] = {5, 6, 7, 11};
+
+// The table set a precision form interpolates in.
+template <class R> struct TabOf;
+template <> struct TabOf<double> {
+  static __device__ __forceinline__ const TableSet<double>& get(const Tables& t) { return t.d; }
+};
+template <> struct TabOf<float> {
+  static __device__ __forceinline__ const TableSet<float>& get(const Tables& t) { return t.f; }
 };
 
 template <class R> struct TabIce { int dumi, dumii, dumjj; R dum1, dum4, dum5; };
@@ -150,46 +173,99 @@ __device__ __forceinline__ int coll_off(int jj, int ii, int i, int j, int k) {
   return ((((jj * rimsize + ii) * isize + i) * rcollsize + j) * collect_table_size) + k;
 }
 
-// apply_table_ice, P3I/p3_table_ice_impl.hpp:175-217
-template <class R>
-__device__ __forceinline__ R apply_table_ice(const double* __restrict__ tab, int idx, const TabIce<R>& t) {
+// Loads the first N values of a 16-byte-aligned record with 16-byte loads.
+template <int N>
+__device__ __forceinline__ void load_rec(const double* __restrict__ r, double (&v)[N]) {
+  static_assert(N % 2 == 0, "records hold pairs");
+#pragma unroll
+  for (int q = 0; q < N; q += 2) {
+    const double2 x = *reinterpret_cast<const double2*>(r + q);
+    v[q] = x.x; v[q + 1] = x.y;
+  }
+}
+template <int N>
+__device__ __forceinline__ void load_rec(const float* __restrict__ r, float (&v)[N]) {
+  static_assert(N % 4 == 0 || N == 2, "records hold quads or pairs");
+  if constexpr (N == 2) {
+    const float2 x = *reinterpret_cast<const float2*>(r);
+    v[0] = x.x; v[1] = x.y;
+  } else {
+#pragma unroll
+    for (int q = 0; q < N; q += 4) {
+      const float4 x = *reinterpret_cast<const float4*>(r + q);
+      v[q] = x.x; v[q + 1] = x.y; v[q + 2] = x.z; v[q + 3] = x.w;
+    }
+  }
+}
+
+// apply_table_ice, P3I/p3_table_ice_impl.hpp:175-217, for the NQ quantities of
+// a repacked record of NR values per (density, rime, size) cell; each value is
+// the reference's trilinear interpolation with its operation order.
+template <int NQ, int NR, class R>
+__device__ __forceinline__ void apply_table_ice_rec(const R* __restrict__ tab, const TabIce<R>& t, R (&out)[NQ],
+                                                    int q0 = 0) {
   const int jj = t.dumjj, ii = t.dumii, i = t.dumi;
   const R w1 = (t.dum1 - R(i) - R(1));
   const R w4 = (t.dum4 - R(ii) - R(1));
-  auto T = [&](int a, int b, int c) { return R(tab[ice_off(a, b, c, idx)]); };
-  R iproc1 = T(jj, ii, i) + w1 * (T(jj, ii, i + 1) - T(jj, ii, i));
-  R gproc1 = T(jj, ii + 1, i) + w1 * (T(jj, ii + 1, i + 1) - T(jj, ii + 1, i));
-  const R tmp1 = iproc1 + w4 * (gproc1 - iproc1);
-  iproc1 = T(jj + 1, ii, i) + w1 * (T(jj + 1, ii, i + 1) - T(jj + 1, ii, i));
-  gproc1 = T(jj + 1, ii + 1, i) + w1 * (T(jj + 1, ii + 1, i + 1) - T(jj + 1, ii + 1, i));
-  const R tmp2 = iproc1 + w4 * (gproc1 - iproc1);
-  return tmp1 + (t.dum5 - R(jj) - R(1)) * (tmp2 - tmp1);
+  const R w5 = (t.dum5 - R(jj) - R(1));
+  // values q0 .. q0+NQ-1 of the record of cell (a, b, i)
+  auto rec = [&](int a, int b) { return tab + ((a * rimsize + b) * isize + i) * NR + q0; };
+  R c[8][NQ];
+  load_rec<NQ>(rec(jj, ii), c[0]);
+  load_rec<NQ>(rec(jj, ii) + NR, c[1]);
+  load_rec<NQ>(rec(jj, ii + 1), c[2]);
+  load_rec<NQ>(rec(jj, ii + 1) + NR, c[3]);
+  load_rec<NQ>(rec(jj + 1, ii), c[4]);
+  load_rec<NQ>(rec(jj + 1, ii) + NR, c[5]);
+  load_rec<NQ>(rec(jj + 1, ii + 1), c[6]);
+  load_rec<NQ>(rec(jj + 1, ii + 1) + NR, c[7]);
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    R iproc1 = c[0][q] + w1 * (c[1][q] - c[0][q]);
+    R gproc1 = c[2][q] + w1 * (c[3][q] - c[2][q]);
+    const R tmp1 = iproc1 + w4 * (gproc1 - iproc1);
+    iproc1 = c[4][q] + w1 * (c[5][q] - c[4][q]);
+    gproc1 = c[6][q] + w1 * (c[7][q] - c[6][q]);
+    const R tmp2 = iproc1 + w4 * (gproc1 - iproc1);
+    out[q] = tmp1 + w5 * (tmp2 - tmp1);
+  }
 }
 
-// apply_table_coll, P3I/p3_table_ice_impl.hpp:219-297
+// apply_table_coll, P3I/p3_table_ice_impl.hpp:219-297, for both quantities of
+// the collection table at once (they are adjacent in the reference layout).
 template <class R>
-__device__ __forceinline__ R apply_table_coll(const double* __restrict__ tab, int idx, const TabIce<R>& ti,
-                                              const TabRain<R>& tr) {
+__device__ __forceinline__ void apply_table_coll2(const R* __restrict__ tab, const TabIce<R>& ti,
+                                                  const TabRain<R>& tr, R (&out)[2]) {
   const int jj = ti.dumjj, ii = ti.dumii, i = ti.dumi, j = tr.dumj;
   const R w1 = (ti.dum1 - R(i) - R(1));
   const R w3 = (tr.dum3 - R(j) - R(1));
   const R w4 = (ti.dum4 - R(ii) - R(1));
-  auto T = [&](int a, int b, int c, int d) { return R(tab[coll_off(a, b, c, d, idx)]); };
-  R dproc1 = T(jj, ii, i, j) + w1 * (T(jj, ii, i + 1, j) - T(jj, ii, i, j));
-  R dproc2 = T(jj, ii, i, j + 1) + w1 * (T(jj, ii, i + 1, j + 1) - T(jj, ii, i, j + 1));
-  R iproc1 = dproc1 + w3 * (dproc2 - dproc1);
-  dproc1 = T(jj, ii + 1, i, j) + w1 * (T(jj, ii + 1, i + 1, j) - T(jj, ii + 1, i, j));
-  dproc2 = T(jj, ii + 1, i, j + 1) + w1 * (T(jj, ii + 1, i + 1, j + 1) - T(jj, ii + 1, i, j + 1));
-  R gproc1 = dproc1 + w3 * (dproc2 - dproc1);
-  const R tmp1 = iproc1 + w4 * (gproc1 - iproc1);
-  dproc1 = T(jj + 1, ii, i, j) + w1 * (T(jj + 1, ii, i + 1, j) - T(jj + 1, ii, i, j));
-  dproc2 = T(jj + 1, ii, i, j + 1) + w1 * (T(jj + 1, ii, i + 1, j + 1) - T(jj + 1, ii, i, j + 1));
-  iproc1 = dproc1 + w3 * (dproc2 - dproc1);
-  dproc1 = T(jj + 1, ii + 1, i, j) + w1 * (T(jj + 1, ii + 1, i + 1, j) - T(jj + 1, ii + 1, i, j));
-  dproc2 = T(jj + 1, ii + 1, i, j + 1) + w1 * (T(jj + 1, ii + 1, i + 1, j + 1) - T(jj + 1, ii + 1, i, j + 1));
-  gproc1 = dproc1 + w3 * (dproc2 - dproc1);
-  const R tmp2 = iproc1 + w4 * (gproc1 - iproc1);
-  return tmp1 + (ti.dum5 - R(jj) - R(1)) * (tmp2 - tmp1);
+  const R w5 = (ti.dum5 - R(jj) - R(1));
+  R c[16][2];
+#pragma unroll
+  for (int n = 0; n < 16; ++n) {
+    // corner bits: 8 = density, 4 = rime, 2 = size, 1 = rain size
+    load_rec<2>(tab + coll_off(jj + (n >> 3), ii + ((n >> 2) & 1), i + ((n >> 1) & 1), j + (n & 1), 0), c[n]);
+  }
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    auto T = [&](int a, int b, int x, int y) { return c[a * 8 + b * 4 + x * 2 + y][q]; };
+    R dproc1 = T(0, 0, 0, 0) + w1 * (T(0, 0, 1, 0) - T(0, 0, 0, 0));
+    R dproc2 = T(0, 0, 0, 1) + w1 * (T(0, 0, 1, 1) - T(0, 0, 0, 1));
+    R iproc1 = dproc1 + w3 * (dproc2 - dproc1);
+    dproc1 = T(0, 1, 0, 0) + w1 * (T(0, 1, 1, 0) - T(0, 1, 0, 0));
+    dproc2 = T(0, 1, 0, 1) + w1 * (T(0, 1, 1, 1) - T(0, 1, 0, 1));
+    R gproc1 = dproc1 + w3 * (dproc2 - dproc1);
+    const R tmp1 = iproc1 + w4 * (gproc1 - iproc1);
+    dproc1 = T(1, 0, 0, 0) + w1 * (T(1, 0, 1, 0) - T(1, 0, 0, 0));
+    dproc2 = T(1, 0, 0, 1) + w1 * (T(1, 0, 1, 1) - T(1, 0, 0, 1));
+    iproc1 = dproc1 + w3 * (dproc2 - dproc1);
+    dproc1 = T(1, 1, 0, 0) + w1 * (T(1, 1, 1, 0) - T(1, 1, 0, 0));
+    dproc2 = T(1, 1, 0, 1) + w1 * (T(1, 1, 1, 1) - T(1, 1, 0, 1));
+    gproc1 = dproc1 + w3 * (dproc2 - dproc1);
+    const R tmp2 = iproc1 + w4 * (gproc1 - iproc1);
+    out[q] = tmp1 + w5 * (tmp2 - tmp1);
+  }
 }
 
 // lookup (Table3), P3I/p3_table3_impl.hpp:14-62
@@ -231,13 +307,31 @@ __device__ __forceinline__ Tab3<R> lookup3(R mu_r, R lamr) {
 
 // apply_table, P3I/p3_table3_impl.hpp:64-83 ([300][10] table)
 template <class R>
-__device__ __forceinline__ R apply_table3(const double* __restrict__ table, const Tab3<R>& t) {
+__device__ __forceinline__ R apply_table3(const R* __restrict__ table, const Tab3<R>& t) {
   const R rdumii_m_dumii = t.rdumii - R(t.dumii);
-  const R t_im1_jm1 = R(table[(t.dumii - 1) * VTABLE_DIM1 + (t.dumjj - 1)]);
-  const R dum1 = (t_im1_jm1 + rdumii_m_dumii * (R(table[t.dumii * VTABLE_DIM1 + (t.dumjj - 1)]) - t_im1_jm1));
-  const R t_im1_j = R(table[(t.dumii - 1) * VTABLE_DIM1 + t.dumjj]);
-  const R dum2 = (t_im1_j + rdumii_m_dumii * (R(table[t.dumii * VTABLE_DIM1 + t.dumjj]) - t_im1_j));
+  const R t_im1_jm1 = table[(t.dumii - 1) * VTABLE_DIM1 + (t.dumjj - 1)];
+  const R dum1 = (t_im1_jm1 + rdumii_m_dumii * (table[t.dumii * VTABLE_DIM1 + (t.dumjj - 1)] - t_im1_jm1));
+  const R t_im1_j = table[(t.dumii - 1) * VTABLE_DIM1 + t.dumjj];
+  const R dum2 = (t_im1_j + rdumii_m_dumii * (table[t.dumii * VTABLE_DIM1 + t.dumjj] - t_im1_j));
   return dum1 + (t.rdumjj - R(t.dumjj)) * (dum2 - dum1);
+}
+
+// apply_table for the mass- and number-weighted rain fall speeds together,
+// from the [300][10][2] (vm, vn) record table: out = {vm, vn}.
+template <class R>
+__device__ __forceinline__ void apply_table3_v(const R* __restrict__ table, const Tab3<R>& t, R (&out)[2]) {
+  const R rdumii_m_dumii = t.rdumii - R(t.dumii);
+  R a[2], b[2], c[2], d[2];
+  load_rec<2>(table + ((t.dumii - 1) * VTABLE_DIM1 + (t.dumjj - 1)) * 2, a);
+  load_rec<2>(table + (t.dumii * VTABLE_DIM1 + (t.dumjj - 1)) * 2, b);
+  load_rec<2>(table + ((t.dumii - 1) * VTABLE_DIM1 + t.dumjj) * 2, c);
+  load_rec<2>(table + (t.dumii * VTABLE_DIM1 + t.dumjj) * 2, d);
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const R dum1 = (a[q] + rdumii_m_dumii * (b[q] - a[q]));
+    const R dum2 = (c[q] + rdumii_m_dumii * (d[q] - c[q]));
+    out[q] = dum1 + (t.rdumjj - R(t.dumjj)) * (dum2 - dum1);
+  }
 }
 
 // ---------------------------------------------------------------- size distributions

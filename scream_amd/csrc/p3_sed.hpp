@@ -114,21 +114,22 @@ __device__ __forceinline__ void sediment(const KParams& p, int i, int kbase, con
         const double rho = g.rho[t];
         if (SP == 0) {
           // cloud_sedimentation, P3I/p3_cloud_sed_impl.hpp:85-112
-          const double cfl = cf[t];
-          const double qc = q[0][t], nc = q[1][t];
-          double qc_incld, nc_incld;
+          // in-cloud values in the arithmetic type of the form (the state stays FP64)
+          const R cfl = R(cf[t]);
+          const R qc = R(q[0][t]), nc = R(q[1][t]);
+          R qc_incld, nc_incld;
           if (first) {
-            const double icfl = 1 / cfl;
-            if (qc >= QSMALL) { qc_incld = vmin(qc * icfl, incloud_limit); nc_incld = vmax(nc * icfl, 0.0); }
+            const R icfl = R(1) / cfl;
+            if (q[0][t] >= QSMALL) { qc_incld = vmin(qc * icfl, R(incloud_limit)); nc_incld = vmax(nc * icfl, R(0)); }
             else { qc_incld = 0; nc_incld = 0; }
           } else {
             qc_incld = qc / cfl;
             nc_incld = nc / cfl;
           }
-          if (qc_incld > QSMALL) {
-            R ncr = R(nc_incld), mu_c, lamc, cdist, cdist1;
-            get_cloud_dsd2<R>(R(qc_incld), ncr, mu_c, R(rho), lamc, cdist, cdist1);
-            q[1][t] = double(ncr) * cfl;
+          if (qc_incld > R(QSMALL)) {
+            R ncr = nc_incld, mu_c, lamc, cdist, cdist1;
+            get_cloud_dsd2<R>(qc_incld, ncr, mu_c, R(rho), lamc, cdist, cdist1);
+            q[1][t] = double(ncr) * cf[t];
             const R dum = R(1) / m_pow(lamc, R(bcn));
             const R acn = R(coef[t]);
             V1[t] = double(acn * m_tgamma(R(4 + bcn) + mu_c) * dum / m_tgamma(mu_c + R(4)));
@@ -136,36 +137,38 @@ __device__ __forceinline__ void sediment(const KParams& p, int i, int kbase, con
           }
         } else if (SP == 1) {
           // rain_sedimentation + compute_rain_fall_velocity, P3I/p3_rain_sed_impl.hpp:14-34,104-126
-          const double cfr = cf[t];
-          const double qr = q[0][t], nr = q[1][t];
-          double qr_incld, nr_incld;
+          const R cfr = R(cf[t]);
+          const R qr = R(q[0][t]), nr = R(q[1][t]);
+          R qr_incld, nr_incld;
           if (first) {
-            const double icfr = 1 / cfr;
-            if (qr >= QSMALL) { qr_incld = vmin(qr * icfr, precip_limit); nr_incld = vmax(nr * icfr, 0.0); }
+            const R icfr = R(1) / cfr;
+            if (q[0][t] >= QSMALL) { qr_incld = vmin(qr * icfr, R(precip_limit)); nr_incld = vmax(nr * icfr, R(0)); }
             else { qr_incld = 0; nr_incld = 0; }
           } else {
             qr_incld = qr / cfr;
             nr_incld = nr / cfr;
           }
-          if (qr_incld > QSMALL) {
-            R nrr = R(nr_incld), mu_r = 0, lamr;
-            get_rain_dsd2<R>(rt, R(qr_incld), nrr, mu_r, lamr);
+          if (qr_incld > R(QSMALL)) {
+            R nrr = nr_incld, mu_r = 0, lamr;
+            get_rain_dsd2<R>(rt, qr_incld, nrr, mu_r, lamr);
             const Tab3<R> t3 = lookup3<R>(mu_r, lamr);
             const R rhofacr = R(coef[t]);
-            V1[t] = double(apply_table3<R>(p.tab.vm, t3) * rhofacr);
-            V2[t] = double(apply_table3<R>(p.tab.vn, t3) * rhofacr);
-            q[1][t] = double(nrr) * cfr;
+            R v[2];  // vm, vn
+            apply_table3_v<R>(TabOf<R>::get(p.tab).rain_v, t3, v);
+            V1[t] = double(v[0] * rhofacr);
+            V2[t] = double(v[1] * rhofacr);
+            q[1][t] = double(nrr) * cf[t];
           }
         } else {
           // ice_sedimentation, P3I/p3_ice_sed_impl.hpp:134-166
-          const double cfi = cf[t];
-          const double qi = q[0][t], ni = q[1][t], qm = q[2][t], bm = q[3][t];
-          double qi_incld, ni_incld, qm_incld, bm_incld;
+          const R cfi = R(cf[t]);
+          const R qi = R(q[0][t]), ni = R(q[1][t]), qm = R(q[2][t]), bm = R(q[3][t]);
+          R qi_incld, ni_incld, qm_incld, bm_incld;
           if (first) {
-            const double icfi = 1 / cfi, icfl = 1 / cfl_ice[t];
-            if (qi >= QSMALL) { qi_incld = vmin(qi * icfi, incloud_limit); ni_incld = vmax(ni * icfi, 0.0); }
+            const R icfi = R(1) / cfi, icfl = R(1) / R(cfl_ice[t]);
+            if (q[0][t] >= QSMALL) { qi_incld = vmin(qi * icfi, R(incloud_limit)); ni_incld = vmax(ni * icfi, R(0)); }
             else { qi_incld = 0; ni_incld = 0; }
-            if (qi >= QSMALL && qm >= QSMALL) { qm_incld = qm * icfi; bm_incld = vmin(vmax(bm * icfl, 0.0), incloud_limit); }
+            if (q[0][t] >= QSMALL && q[2][t] >= QSMALL) { qm_incld = qm * icfi; bm_incld = vmin(vmax(bm * icfl, R(0)), R(incloud_limit)); }
             else { qm_incld = 0; bm_incld = 0; }
           } else {
             qi_incld = qi / cfi;
@@ -173,21 +176,22 @@ __device__ __forceinline__ void sediment(const KParams& p, int i, int kbase, con
             qm_incld = qm / cfi;
             bm_incld = bm / cfi;
           }
-          if (qi_incld > QSMALL) {
-            R nir = vmax(R(ni_incld), R(NSMALL));
-            R qmr = R(qm_incld), bmr = R(bm_incld);
-            const R qir = R(qi_incld);
+          if (qi_incld > R(QSMALL)) {
+            R nir = vmax(ni_incld, R(NSMALL));
+            R qmr = qm_incld, bmr = bm_incld;
+            const R qir = qi_incld;
             const R rhop = calc_bulk_rho_rime<R>(rt, qir, qmr, bmr);
-            q[2][t] = double(qmr) * cfi;
-            q[3][t] = double(bmr) * cfi;
+            q[2][t] = double(qmr) * cf[t];
+            q[3][t] = double(bmr) * cf[t];
             const TabIce<R> ti = lookup_ice<R>(qir, nir, qmr, rhop);
-            const R t_ni_fall = apply_table_ice<R>(p.tab.ice, 0, ti);
-            const R t_qi_fall = apply_table_ice<R>(p.tab.ice, 1, ti);
-            const R t_lammax = apply_table_ice<R>(p.tab.ice, 6, ti);
-            const R t_lammin = apply_table_ice<R>(p.tab.ice, 7, ti);
+            // quantities 0, 1 then 6, 7 of the records (two halves bound the registers in FP64)
+            R v[2], w[2];
+            apply_table_ice_rec<2, 4, R>(TabOf<R>::get(p.tab).ice_sed, ti, v, 0);
+            apply_table_ice_rec<2, 4, R>(TabOf<R>::get(p.tab).ice_sed, ti, w, 2);
+            const R t_ni_fall = v[0], t_qi_fall = v[1], t_lammax = w[0], t_lammin = w[1];
             nir = vmin(nir, t_lammax * nir);
             nir = vmax(nir, t_lammin * nir);
-            q[1][t] = double(nir) * cfi;
+            q[1][t] = double(nir) * cf[t];
             const R rhofaci = R(coef[t]);
             V1[t] = double(R(rt.ice_sedimentation_factor) * t_qi_fall * rhofaci);
             V2[t] = double(R(rt.ice_sedimentation_factor) * t_ni_fall * rhofaci);
@@ -381,10 +385,9 @@ __device__ __forceinline__ void sed_block(const KParams& p, int bid) {
         if (qi_gt) {
           const TabIce<R> ti = lookup_ice<R>(R(qi_incld), R(ni_incld), R(qm_incld), rhop);
           if (p.dbg_ice_idx) p.dbg_ice_idx[x] = ti.dumjj * 10000 + ti.dumii * 100 + ti.dumi;
-          const R t_eff = apply_table_ice<R>(p.tab.ice, 5, ti);
-          const R t_lammax = apply_table_ice<R>(p.tab.ice, 6, ti);
-          const R t_lammin = apply_table_ice<R>(p.tab.ice, 7, ti);
-          const R t_rhomm = apply_table_ice<R>(p.tab.ice, 11, ti);
+          R v[4];  // quantities 5, 6, 7, 11
+          apply_table_ice_rec<4, 4, R>(TabOf<R>::get(p.tab).ice_p3, ti, v);
+          const R t_eff = v[0], t_lammax = v[1], t_lammin = v[2], t_rhomm = v[3];
           ni_incld = vmin(ni_incld, double(t_lammax) * ni_incld);
           ni_incld = vmax(ni_incld, double(t_lammin) * ni_incld);
           reff_i = double(t_eff);
@@ -432,8 +435,11 @@ __device__ __forceinline__ void sed_block(const KParams& p, int bid) {
   if (err) atomicOr(p.err, err);
 }
 
+// Up to 5 levels per lane the kernel fits two wavefronts per SIMD (256 registers).
+template <int L> constexpr int sed_waves_per_simd() { return L <= 5 ? 2 : 1; }
+
 template <class R, int LPC, int L, int WPB>
-__global__ __launch_bounds__(64 * WPB) void p3_sed_kernel(const KParams p) {
+__global__ __launch_bounds__(64 * WPB, sed_waves_per_simd<L>()) void p3_sed_kernel(const KParams p) {
   sed_block<R, LPC, L, WPB>(p, blockIdx.x);
 }
 

@@ -34,6 +34,56 @@ void default_shape(int precision, int nk, p3d::Shape* sh) {
   sh->lpc = nk <= 16 * 5 ? 16 : (nk <= 32 * 5 ? 32 : 64);
 }
 
+// Offsets (in elements) of the tables in one device allocation; each is a
+// multiple of 4 elements, so every record starts 16-byte aligned.
+namespace TableLayout {
+constexpr size_t kCells = (size_t)p3c::densize * p3c::rimsize * p3c::isize;
+constexpr size_t coll = 0;
+constexpr size_t revap = coll + p3c::COLL_TABLE_LEN;
+constexpr size_t ice_p2 = revap + p3c::RAIN_TABLE_LEN;
+constexpr size_t ice_sed = ice_p2 + kCells * 8;
+constexpr size_t ice_p3 = ice_sed + kCells * 4;
+constexpr size_t rain_v = ice_p3 + kCells * 4;
+constexpr size_t total = rain_v + (size_t)p3c::RAIN_TABLE_LEN * 2;
+static_assert(revap % 4 == 0 && ice_p2 % 4 == 0 && ice_sed % 4 == 0 && ice_p3 % 4 == 0 && rain_v % 4 == 0,
+              "records must stay 16-byte aligned");
+}  // namespace TableLayout
+
+template <class T>
+p3d::TableSet<T> table_set(const T* base) {
+  p3d::TableSet<T> s;
+  s.coll = base + TableLayout::coll;
+  s.revap = base + TableLayout::revap;
+  s.ice_p2 = base + TableLayout::ice_p2;
+  s.ice_sed = base + TableLayout::ice_sed;
+  s.ice_p3 = base + TableLayout::ice_p3;
+  s.rain_v = base + TableLayout::rain_v;
+  return s;
+}
+
+// The device table set in type T from the reference tables: collection and
+// rain-evaporation tables as they are, the ice table repacked into per-kernel
+// records of the quantities each kernel interpolates, and the two rain
+// fall-speed tables interleaved.
+template <class T>
+std::vector<T> pack_tables(const p3x_table_data* t) {
+  using namespace TableLayout;
+  std::vector<T> v(total, T(0));
+  for (size_t j = 0; j < (size_t)p3c::COLL_TABLE_LEN; ++j) v[coll + j] = (T)t->collect_table[j];
+  for (size_t j = 0; j < (size_t)p3c::RAIN_TABLE_LEN; ++j) v[revap + j] = (T)t->revap_table[j];
+  for (size_t cell = 0; cell < kCells; ++cell) {
+    const double* r = t->ice_table + cell * p3c::ice_table_size;
+    for (int q = 0; q < 7; ++q) v[ice_p2 + cell * 8 + q] = (T)r[p3d::kIceP2Quant[q]];
+    for (int q = 0; q < 4; ++q) v[ice_sed + cell * 4 + q] = (T)r[p3d::kIceSedQuant[q]];
+    for (int q = 0; q < 4; ++q) v[ice_p3 + cell * 4 + q] = (T)r[p3d::kIceP3Quant[q]];
+  }
+  for (size_t j = 0; j < (size_t)p3c::RAIN_TABLE_LEN; ++j) {
+    v[rain_v + 2 * j] = (T)t->vm_table[j];
+    v[rain_v + 2 * j + 1] = (T)t->vn_table[j];
+  }
+  return v;
+}
+
 // KParams of one column set; scratch points at SC_COUNT*nk*ld doubles followed
 // by ld ints.
 void fill_params(const p3x_context* c, const p3x_runtime* runtime, const p3x_prognostic_state* ps,
@@ -61,7 +111,8 @@ void fill_params(const p3x_context* c, const p3x_runtime* runtime, const p3x_pro
   k.dbg_status = col_status;
   k.dbg_substeps = sed_substeps;
   k.dbg_ice_idx = ice_table_idx;
-  k.tab.ice = c->d_ice; k.tab.coll = c->d_coll; k.tab.vn = c->d_vn; k.tab.vm = c->d_vm; k.tab.revap = c->d_revap;
+  k.tab.d = table_set(c->d_tab);
+  k.tab.f = table_set(c->d_tab_f);
   static_assert(sizeof(p3d::Runtime) == sizeof(p3x_runtime), "runtime layouts differ");
   std::memcpy(&k.rt, runtime, sizeof(k.rt));
   k.ncol = nj; k.nlev = nk; k.ld = ld;
@@ -147,16 +198,13 @@ extern "C" int p3x_init(const p3x_table_data* t, p3x_context** out) {
   *out = nullptr;
   p3x_context* c = new p3x_context();
   hipError_t e = hipGetDevice(&c->device);
-  auto up = [&](double** dst, const double* src, size_t n) {
-    if (e != hipSuccess) return;
-    e = hipMalloc((void**)dst, n * sizeof(double));
-    if (e == hipSuccess) e = hipMemcpy(*dst, src, n * sizeof(double), hipMemcpyHostToDevice);
+  auto up = [&](auto** dst, const auto& host) {
+    const size_t bytes = host.size() * sizeof(host[0]);
+    if (e == hipSuccess) e = hipMalloc((void**)dst, bytes);
+    if (e == hipSuccess) e = hipMemcpy(*dst, host.data(), bytes, hipMemcpyHostToDevice);
   };
-  up(&c->d_ice, t->ice_table, p3c::ICE_TABLE_LEN);
-  up(&c->d_coll, t->collect_table, p3c::COLL_TABLE_LEN);
-  up(&c->d_vn, t->vn_table, p3c::RAIN_TABLE_LEN);
-  up(&c->d_vm, t->vm_table, p3c::RAIN_TABLE_LEN);
-  up(&c->d_revap, t->revap_table, p3c::RAIN_TABLE_LEN);
+  up(&c->d_tab, pack_tables<double>(t));
+  up(&c->d_tab_f, pack_tables<float>(t));
   if (e == hipSuccess) e = hipMalloc((void**)&c->d_err, sizeof(unsigned));
   if (e == hipSuccess) e = hipMemset(c->d_err, 0, sizeof(unsigned));
   if (e == hipSuccess) e = hipEventCreate(&c->ev0);
@@ -174,11 +222,8 @@ extern "C" void p3x_finalize(p3x_context* c) {
   if (!c) return;
   (void)hipDeviceSynchronize();
   // teardown: nothing useful can be done with a failure here
-  (void)hipFree(c->d_ice);
-  (void)hipFree(c->d_coll);
-  (void)hipFree(c->d_vn);
-  (void)hipFree(c->d_vm);
-  (void)hipFree(c->d_revap);
+  (void)hipFree(c->d_tab);
+  (void)hipFree(c->d_tab_f);
   (void)hipFree(c->d_scratch);
   (void)hipFree(c->d_err);
   if (c->ev0) (void)hipEventDestroy(c->ev0);

@@ -13,11 +13,8 @@
 struct p3x_context {
   int device = 0;
   int ncu = 0;
-  double* d_ice = nullptr;
-  double* d_coll = nullptr;
-  double* d_vn = nullptr;
-  double* d_vm = nullptr;
-  double* d_revap = nullptr;
+  double* d_tab = nullptr;   // the device table set in FP64 (layout: TableLayout), one allocation
+  float* d_tab_f = nullptr;  // the same in FP32 for the mixed-precision form
   double* d_scratch = nullptr;
   size_t scratch_bytes = 0;
   unsigned* d_err = nullptr;
