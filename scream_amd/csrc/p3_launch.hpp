@@ -21,7 +21,10 @@ struct Shape {
 // Largest levels-per-lane count of the sedimentation kernel.
 constexpr int kMaxLevelsPerLane = 8;
 // Wavefronts per block of the sedimentation kernel.
-constexpr int kSedWaves = 4;
+#ifndef P3_SED_WAVES
+#define P3_SED_WAVES 4
+#endif
+constexpr int kSedWaves = P3_SED_WAVES;
 
 // Levels per lane the sedimentation kernel is instantiated with for nlev
 // levels on lpc lanes (0 if nlev does not fit).

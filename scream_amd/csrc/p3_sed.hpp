@@ -184,11 +184,17 @@ __device__ __forceinline__ void sediment(const KParams& p, int i, int kbase, con
             q[2][t] = double(qmr) * cf[t];
             q[3][t] = double(bmr) * cf[t];
             const TabIce<R> ti = lookup_ice<R>(qir, nir, qmr, rhop);
-            // quantities 0, 1 then 6, 7 of the records (two halves bound the registers in FP64)
-            R v[2], w[2];
-            apply_table_ice_rec<2, 4, R>(TabOf<R>::get(p.tab).ice_sed, ti, v, 0);
-            apply_table_ice_rec<2, 4, R>(TabOf<R>::get(p.tab).ice_sed, ti, w, 2);
-            const R t_ni_fall = v[0], t_qi_fall = v[1], t_lammax = w[0], t_lammin = w[1];
+            // quantities 0, 1, 6, 7 of the records (in FP64 as two halves, which bounds the registers)
+            R v[4];
+            if constexpr (sizeof(R) == 4) {
+              apply_table_ice_rec<4, 4, R>(TabOf<R>::get(p.tab).ice_sed, ti, v, 0);
+            } else {
+              R a[2], b[2];
+              apply_table_ice_rec<2, 4, R>(TabOf<R>::get(p.tab).ice_sed, ti, a, 0);
+              apply_table_ice_rec<2, 4, R>(TabOf<R>::get(p.tab).ice_sed, ti, b, 2);
+              v[0] = a[0]; v[1] = a[1]; v[2] = b[0]; v[3] = b[1];
+            }
+            const R t_ni_fall = v[0], t_qi_fall = v[1], t_lammax = v[2], t_lammin = v[3];
             nir = vmin(nir, t_lammax * nir);
             nir = vmax(nir, t_lammin * nir);
             q[1][t] = double(nir) * cf[t];
@@ -262,175 +268,199 @@ __device__ __forceinline__ void sediment(const KParams& p, int i, int kbase, con
   nsub_out = nsub;
 }
 
-// One block's work: WPB wavefronts of 64/LPC columns each, starting at column
-// bid * WPB * 64/LPC of the column set p.
+// homogeneous_freezing (P3I/p3_ice_sed_impl.hpp:207-270) + p3_main_part3
+// (P3I/p3_main_impl_part3.hpp:60-199, only the fields p3_main outputs) for
+// level k of column i, after sedimentation.
+template <class R>
+__device__ __forceinline__ void part3_level(const KParams& p, int k, int i, bool active) {
+  const int nlev = p.nlev;
+  const size_t ld = (size_t)p.ld;
+  const size_t x = (size_t)k * ld + i;
+  const double latvap = LatVap, latice = LatIce, inv_cp = INV_CP;
+  const Runtime& rt = p.rt;
+  p.precip_ice_flux[x] = 0;
+  if (k == nlev - 1) p.precip_ice_flux[x + ld] = 0;
+  double reff_c = 10.e-6, reff_i = 25.e-6, reff_r = 500.e-6, rho_qi = 0;
+  if (active) {
+    double qc = p.qc[x], nc = p.nc[x], qr = p.qr[x], nr = p.nr[x], qi = p.qi[x], ni = p.ni[x];
+    double qm = p.qm[x], bm = p.bm[x], qv = p.qv[x], th = p.th[x];
+    const double inv_exner = p.inv_exner[x];
+    if (rt.do_ice_production) {
+      const bool t_lt = p.scratch[((size_t)SC_T * nlev + k) * ld + i] < T_homogfrz;
+      const bool qc_ge = t_lt && qc >= QSMALL;
+      const bool qr_ge = t_lt && qr >= QSMALL;
+      const double Qc_nuc = qc, Qr_nuc = qr;
+      const double Nc_nuc = vmax(nc, NSMALL), Nr_nuc = vmax(nr, NSMALL);
+      if (qc_ge) {
+        qm = qm + Qc_nuc; qi = qi + Qc_nuc; bm = bm + Qc_nuc * INV_RHO_RIMEMAX; ni = ni + Nc_nuc;
+        th = th + inv_exner * Qc_nuc * latice * inv_cp;
+      }
+      if (qr_ge) {
+        qm = qm + Qr_nuc; qi = qi + Qr_nuc; bm = bm + Qr_nuc * INV_RHO_RIMEMAX; ni = ni + Nr_nuc;
+        th = th + inv_exner * Qr_nuc * latice * inv_cp;
+      }
+      if (qc_ge) { qc = 0; nc = 0; }
+      if (qr_ge) { qr = 0; nr = 0; }
+    }
+    // p3_main_part3, P3I/p3_main_impl_part3.hpp:60-199 (only the fields p3_main outputs)
+    const double rho = p.dpres[x] / p.dz[x] / gravit;
+    const double inv_rho = 1 / rho;
+    const double cfl = p.cld_frac_l[x], cfr = p.cld_frac_r[x], cfi = p.cld_frac_i[x];
+    double vap_liq = p.vap_liq_exchange[x];
+    // cloud
+    if (qc >= QSMALL) {
+      R ncr = R(nc / cfl), mu_c, lamc, cd, cd1;
+      get_cloud_dsd2<R>(R(qc / cfl), ncr, mu_c, R(rho), lamc, cd, cd1);
+      nc = double(ncr) * cfl;
+      reff_c = double(R(0.5) * (mu_c + R(3)) / lamc);
+    } else {
+      qv = qv + qc;
+      th = th - inv_exner * qc * latvap * inv_cp;
+      vap_liq = vap_liq - qc;
+      qc = 0; nc = 0;
+    }
+    // rain
+    if (qr >= QSMALL) {
+      R nrr = R(nr / cfr), mu_r = 0, lamr;
+      get_rain_dsd2<R>(rt, R(qr / cfr), nrr, mu_r, lamr);
+      nr = double(nrr) * cfr;
+      reff_r = double(R(1.5) / lamr);
+    } else {
+      qv = qv + qr;
+      th = th - inv_exner * qr * latvap * inv_cp;
+      vap_liq = vap_liq - qr;
+      qr = 0; nr = 0;
+    }
+    // ice
+    {
+      const bool qi_gt = qi >= QSMALL;
+      ni = vmax(ni, NSMALL);
+      const double qi_incld = qi / cfi;
+      double ni_incld = ni / cfi;
+      R qmr = R(qm / cfi), bmr = R(bm / cfi);
+      double qm_incld = qm / cfi, bm_incld = bm / cfi;
+      R rhop = 0;
+      if (qi_gt) {
+        rhop = calc_bulk_rho_rime<R>(rt, R(qi_incld), qmr, bmr);
+        qm_incld = double(qmr); bm_incld = double(bmr);
+        qm = qm_incld * cfi;
+        bm = bm_incld * cfi;
+      }
+      impose_max_total_ni<double>(ni_incld, rt.max_total_ni, inv_rho);
+      if (qi_gt) {
+        const TabIce<R> ti = lookup_ice<R>(R(qi_incld), R(ni_incld), R(qm_incld), rhop);
+        if (p.dbg_ice_idx) p.dbg_ice_idx[x] = ti.dumjj * 10000 + ti.dumii * 100 + ti.dumi;
+        R v[4];  // quantities 5, 6, 7, 11
+        apply_table_ice_rec<4, 4, R>(TabOf<R>::get(p.tab).ice_p3, ti, v);
+        const R t_eff = v[0], t_lammax = v[1], t_lammin = v[2], t_rhomm = v[3];
+        ni_incld = vmin(ni_incld, double(t_lammax) * ni_incld);
+        ni_incld = vmax(ni_incld, double(t_lammin) * ni_incld);
+        reff_i = double(t_eff);
+        rho_qi = double(t_rhomm);
+      } else if (p.dbg_ice_idx) {
+        p.dbg_ice_idx[x] = -1;
+      }
+      ni = ni_incld * cfi;
+      if (qm < QSMALL && qi_gt) { qm = 0; bm = 0; }
+      if (!qi_gt) {
+        qv = qv + qi;
+        th = th - inv_exner * qi * (latvap + latice) * inv_cp;
+        qi = 0; ni = 0; qm = 0; bm = 0;
+      }
+    }
+    if (qr < QSMALL) nr = 0;
+    p.qc[x] = qc; p.nc[x] = nc; p.qr[x] = qr; p.nr[x] = nr; p.qi[x] = qi;
+    p.ni[x] = ni; p.qm[x] = qm; p.bm[x] = bm; p.qv[x] = qv; p.th[x] = th;
+    p.vap_liq_exchange[x] = vap_liq;
+  } else if (p.dbg_ice_idx) {
+    p.dbg_ice_idx[x] = -1;
+  }
+  p.diag_eff_radius_qc[x] = reff_c;
+  p.diag_eff_radius_qi[x] = reff_i;
+  p.diag_eff_radius_qr[x] = reff_r;
+  p.rho_qi[x] = rho_qi;
+  p.diag_eff_radius_qc[x] = reff_c;
+  p.diag_eff_radius_qi[x] = reff_i;
+  p.diag_eff_radius_qr[x] = reff_r;
+  p.rho_qi[x] = rho_qi;
+}
+
+// One block's work: WPB wavefronts of 64/LPC columns each, the CB = WPB * 64/LPC
+// columns starting at column bid * CB of the column set p.  Sedimentation runs
+// with LPC lanes per column; part 3, which has no vertical dependence, then
+// runs with the block's lanes over consecutive columns of a level, so that its
+// loads and stores cover whole cache-line segments.
 template <class R, int LPC, int L, int WPB>
 __device__ __forceinline__ void sed_block(const KParams& p, int bid) {
   constexpr int CPW = 64 / LPC;  // columns per wavefront
+  constexpr int CB = WPB * CPW;  // columns per block
   static_assert(64 % LPC == 0, "a column group must divide the wavefront");
+  static_assert((CB & (CB - 1)) == 0, "columns per block must be a power of two");
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int j = lane % LPC;
-  const int i = (bid * WPB + wave) * CPW + lane / LPC;
-  // Whole groups leave together: the cross-lane operations below stay inside a group.
-  if (i >= p.ncol) return;
+  const int col0 = bid * CB;
+  const int i = col0 + wave * CPW + lane / LPC;
   const int nlev = p.nlev, kbot = nlev - 1;
   const int kbase = j * L;
   const size_t ld = (size_t)p.ld;
-  const double dt = p.dt, inv_dt = 1 / dt;
-  const double latvap = LatVap, latice = LatIce, inv_cp = INV_CP;
-  const Runtime& rt = p.rt;
-  const bool active = p.colstat[i] == 2;
   unsigned err = 0;
 
-  double pflux[L];
+  // Whole groups skip together: the cross-lane operations below stay inside a group.
+  if (i < p.ncol) {
+    const double inv_dt = 1 / p.dt;
+    const bool active = p.colstat[i] == 2;
+    double pflux[L];
 #pragma unroll
-  for (int t = 0; t < L; ++t) pflux[t] = 0;
-  double prt_c = 0, prt_r = 0, prt_i = 0;
-  int ns_c = 0, ns_r = 0, ns_i = 0;
-  bool pres_c = false, pres_r = false, pres_i = false;
-
-  if (active) {
-    Geo<L> g;
+    for (int t = 0; t < L; ++t) pflux[t] = 0;
+    double prt_c = 0, prt_r = 0, prt_i = 0;
+    int ns_c = 0, ns_r = 0, ns_i = 0;
+    bool pres_c = false, pres_r = false, pres_i = false;
+    if (active) {
+      Geo<L> g;
+#pragma unroll
+      for (int t = 0; t < L; ++t) {
+        const int k = kbase + t;
+        const size_t x = (size_t)(k < nlev ? k : 0) * ld + i;
+        const double dz = k < nlev ? p.dz[x] : 1.0;
+        const double dp = k < nlev ? p.dpres[x] : 1.0;
+        g.rho[t] = dp / dz / gravit;
+        g.inv_rho[t] = 1 / g.rho[t];
+        g.inv_dz[t] = 1 / dz;
+      }
+      sediment<0, R, LPC, L>(p, i, kbase, g, pflux, prt_c, ns_c, pres_c, err);
+      sediment<1, R, LPC, L>(p, i, kbase, g, pflux, prt_r, ns_r, pres_r, err);
+      sediment<2, R, LPC, L>(p, i, kbase, g, pflux, prt_i, ns_i, pres_i, err);
+    }
+    // rain flux at the interface below each level (rain_sed, P3I/p3_rain_sed_impl.hpp:140-153)
 #pragma unroll
     for (int t = 0; t < L; ++t) {
       const int k = kbase + t;
-      const size_t x = (size_t)(k < nlev ? k : 0) * ld + i;
-      const double dz = k < nlev ? p.dz[x] : 1.0;
-      const double dp = k < nlev ? p.dpres[x] : 1.0;
-      g.rho[t] = dp / dz / gravit;
-      g.inv_rho[t] = 1 / g.rho[t];
-      g.inv_dz[t] = 1 / dz;
+      if (k >= nlev) continue;
+      p.precip_liq_flux[(size_t)(k + 1) * ld + i] = pflux[t];
+      if (k == kbot) {
+        double liq = 0, ice = 0;
+        if (pres_c) liq = prt_c * INV_RHO_H2O * inv_dt;
+        if (pres_r) liq += prt_r * INV_RHO_H2O * inv_dt;
+        if (pres_i) ice += prt_i * INV_RHO_H2O * inv_dt;
+        p.precip_liq_surf[i] = liq;
+        p.precip_ice_surf[i] = ice;
+      }
     }
-    sediment<0, R, LPC, L>(p, i, kbase, g, pflux, prt_c, ns_c, pres_c, err);
-    sediment<1, R, LPC, L>(p, i, kbase, g, pflux, prt_r, ns_r, pres_r, err);
-    sediment<2, R, LPC, L>(p, i, kbase, g, pflux, prt_i, ns_i, pres_i, err);
-  }
-
-  // ---- homogeneous_freezing (P3I/p3_ice_sed_impl.hpp:207-270) + p3_main_part3
-#pragma unroll
-  for (int t = 0; t < L; ++t) {
-    const int k = kbase + t;
-    if (k >= nlev) continue;
-    const size_t x = (size_t)k * ld + i;
-    p.precip_liq_flux[x + ld] = pflux[t];
-    p.precip_ice_flux[x] = 0;
-    double reff_c = 10.e-6, reff_i = 25.e-6, reff_r = 500.e-6, rho_qi = 0;
-    if (active) {
-      double qc = p.qc[x], nc = p.nc[x], qr = p.qr[x], nr = p.nr[x], qi = p.qi[x], ni = p.ni[x];
-      double qm = p.qm[x], bm = p.bm[x], qv = p.qv[x], th = p.th[x];
-      const double inv_exner = p.inv_exner[x];
-      if (rt.do_ice_production) {
-        const bool t_lt = p.scratch[((size_t)SC_T * nlev + k) * ld + i] < T_homogfrz;
-        const bool qc_ge = t_lt && qc >= QSMALL;
-        const bool qr_ge = t_lt && qr >= QSMALL;
-        const double Qc_nuc = qc, Qr_nuc = qr;
-        const double Nc_nuc = vmax(nc, NSMALL), Nr_nuc = vmax(nr, NSMALL);
-        if (qc_ge) {
-          qm = qm + Qc_nuc; qi = qi + Qc_nuc; bm = bm + Qc_nuc * INV_RHO_RIMEMAX; ni = ni + Nc_nuc;
-          th = th + inv_exner * Qc_nuc * latice * inv_cp;
-        }
-        if (qr_ge) {
-          qm = qm + Qr_nuc; qi = qi + Qr_nuc; bm = bm + Qr_nuc * INV_RHO_RIMEMAX; ni = ni + Nr_nuc;
-          th = th + inv_exner * Qr_nuc * latice * inv_cp;
-        }
-        if (qc_ge) { qc = 0; nc = 0; }
-        if (qr_ge) { qr = 0; nr = 0; }
+    if (j == 0) {
+      p.precip_liq_flux[i] = 0;  // the top interface
+      if (p.dbg_substeps) {
+        p.dbg_substeps[i] = ns_c;
+        p.dbg_substeps[(size_t)p.ncol + i] = ns_r;
+        p.dbg_substeps[(size_t)2 * p.ncol + i] = ns_i;
       }
-      // p3_main_part3, P3I/p3_main_impl_part3.hpp:60-199 (only the fields p3_main outputs)
-      const double rho = p.dpres[x] / p.dz[x] / gravit;
-      const double inv_rho = 1 / rho;
-      const double cfl = p.cld_frac_l[x], cfr = p.cld_frac_r[x], cfi = p.cld_frac_i[x];
-      double vap_liq = p.vap_liq_exchange[x];
-      // cloud
-      if (qc >= QSMALL) {
-        R ncr = R(nc / cfl), mu_c, lamc, cd, cd1;
-        get_cloud_dsd2<R>(R(qc / cfl), ncr, mu_c, R(rho), lamc, cd, cd1);
-        nc = double(ncr) * cfl;
-        reff_c = double(R(0.5) * (mu_c + R(3)) / lamc);
-      } else {
-        qv = qv + qc;
-        th = th - inv_exner * qc * latvap * inv_cp;
-        vap_liq = vap_liq - qc;
-        qc = 0; nc = 0;
-      }
-      // rain
-      if (qr >= QSMALL) {
-        R nrr = R(nr / cfr), mu_r = 0, lamr;
-        get_rain_dsd2<R>(rt, R(qr / cfr), nrr, mu_r, lamr);
-        nr = double(nrr) * cfr;
-        reff_r = double(R(1.5) / lamr);
-      } else {
-        qv = qv + qr;
-        th = th - inv_exner * qr * latvap * inv_cp;
-        vap_liq = vap_liq - qr;
-        qr = 0; nr = 0;
-      }
-      // ice
-      {
-        const bool qi_gt = qi >= QSMALL;
-        ni = vmax(ni, NSMALL);
-        const double qi_incld = qi / cfi;
-        double ni_incld = ni / cfi;
-        R qmr = R(qm / cfi), bmr = R(bm / cfi);
-        double qm_incld = qm / cfi, bm_incld = bm / cfi;
-        R rhop = 0;
-        if (qi_gt) {
-          rhop = calc_bulk_rho_rime<R>(rt, R(qi_incld), qmr, bmr);
-          qm_incld = double(qmr); bm_incld = double(bmr);
-          qm = qm_incld * cfi;
-          bm = bm_incld * cfi;
-        }
-        impose_max_total_ni<double>(ni_incld, rt.max_total_ni, inv_rho);
-        if (qi_gt) {
-          const TabIce<R> ti = lookup_ice<R>(R(qi_incld), R(ni_incld), R(qm_incld), rhop);
-          if (p.dbg_ice_idx) p.dbg_ice_idx[x] = ti.dumjj * 10000 + ti.dumii * 100 + ti.dumi;
-          R v[4];  // quantities 5, 6, 7, 11
-          apply_table_ice_rec<4, 4, R>(TabOf<R>::get(p.tab).ice_p3, ti, v);
-          const R t_eff = v[0], t_lammax = v[1], t_lammin = v[2], t_rhomm = v[3];
-          ni_incld = vmin(ni_incld, double(t_lammax) * ni_incld);
-          ni_incld = vmax(ni_incld, double(t_lammin) * ni_incld);
-          reff_i = double(t_eff);
-          rho_qi = double(t_rhomm);
-        } else if (p.dbg_ice_idx) {
-          p.dbg_ice_idx[x] = -1;
-        }
-        ni = ni_incld * cfi;
-        if (qm < QSMALL && qi_gt) { qm = 0; bm = 0; }
-        if (!qi_gt) {
-          qv = qv + qi;
-          th = th - inv_exner * qi * (latvap + latice) * inv_cp;
-          qi = 0; ni = 0; qm = 0; bm = 0;
-        }
-      }
-      if (qr < QSMALL) nr = 0;
-      p.qc[x] = qc; p.nc[x] = nc; p.qr[x] = qr; p.nr[x] = nr; p.qi[x] = qi;
-      p.ni[x] = ni; p.qm[x] = qm; p.bm[x] = bm; p.qv[x] = qv; p.th[x] = th;
-      p.vap_liq_exchange[x] = vap_liq;
-    } else if (p.dbg_ice_idx) {
-      p.dbg_ice_idx[x] = -1;
-    }
-    p.diag_eff_radius_qc[x] = reff_c;
-    p.diag_eff_radius_qi[x] = reff_i;
-    p.diag_eff_radius_qr[x] = reff_r;
-    p.rho_qi[x] = rho_qi;
-    if (k == kbot) {
-      p.precip_ice_flux[x + ld] = 0;
-      double liq = 0, ice = 0;
-      if (pres_c) liq = prt_c * INV_RHO_H2O * inv_dt;
-      if (pres_r) liq += prt_r * INV_RHO_H2O * inv_dt;
-      if (pres_i) ice += prt_i * INV_RHO_H2O * inv_dt;
-      p.precip_liq_surf[i] = liq;
-      p.precip_ice_surf[i] = ice;
     }
   }
-  if (j == 0) {
-    p.precip_liq_flux[i] = 0;  // the top interface
-    if (p.dbg_substeps) {
-      p.dbg_substeps[i] = ns_c;
-      p.dbg_substeps[(size_t)p.ncol + i] = ns_r;
-      p.dbg_substeps[(size_t)2 * p.ncol + i] = ns_i;
-    }
+  // the sedimented fields, stored above by other lanes of the block, are read below
+  __syncthreads();
+  for (int e = threadIdx.x; e < CB * nlev; e += 64 * WPB) {
+    const int k = e / CB, c = e % CB, ic = col0 + c;
+    if (ic < p.ncol) part3_level<R>(p, k, ic, p.colstat[ic] == 2);
   }
   if (err) atomicOr(p.err, err);
 }
@@ -438,9 +468,19 @@ __device__ __forceinline__ void sed_block(const KParams& p, int bid) {
 // Up to 5 levels per lane the kernel fits two wavefronts per SIMD (256 registers).
 template <int L> constexpr int sed_waves_per_simd() { return L <= 5 ? 2 : 1; }
 
+// Workgroups go to the 8 XCDs round-robin, and each XCD has its own L2.  A
+// block's lanes read a few columns of many levels, so neighbouring blocks share
+// every cache line they touch: give each XCD a contiguous range of blocks, so
+// that the blocks sharing lines run on one XCD at about the same time.
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+  constexpr int kXcds = 8;
+  const int q = nb / kXcds, r = nb % kXcds, x = b % kXcds, n = b / kXcds;
+  return x * q + (x < r ? x : r) + n;
+}
+
 template <class R, int LPC, int L, int WPB>
 __global__ __launch_bounds__(64 * WPB, sed_waves_per_simd<L>()) void p3_sed_kernel(const KParams p) {
-  sed_block<R, LPC, L, WPB>(p, blockIdx.x);
+  sed_block<R, LPC, L, WPB>(p, xcd_block(blockIdx.x, gridDim.x));
 }
 
 }  // namespace p3d

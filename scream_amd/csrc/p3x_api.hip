@@ -24,14 +24,14 @@ int dev_fail(hipError_t e) { return e == hipErrorOutOfMemory ? P3X_ERR_NO_MEMORY
 // Default launch shape per (precision, nlev), from sweeps on MI355X at 21600x72
 // and 16384x128 in both precisions: 64 columns x 256 threads for part 1 + 2
 // (one wavefront per SIMD: the two-wave variant's spills cost more than the
-// occupancy gains), and the fewest sedimentation lanes that keep at most 5
-// levels per lane (more levels per lane push the kernel to one wavefront per SIMD).
+// occupancy gains), and 32 sedimentation lanes per column up to 160 levels
+// (at most 5 levels per lane; more push the kernel to one wavefront per SIMD).
 void default_shape(int precision, int nk, p3d::Shape* sh) {
   (void)precision;
   sh->cb = 64;
   sh->nt = 256;
   sh->wpe = 1;
-  sh->lpc = nk <= 16 * 5 ? 16 : (nk <= 32 * 5 ? 32 : 64);
+  sh->lpc = nk <= 48 ? 16 : (nk <= 32 * 5 ? 32 : 64);
 }
 
 // Offsets (in elements) of the tables in one device allocation; each is a
