@@ -26,10 +26,18 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", f"--offload-arch={A
 SOURCES = ["p3_kernel_fp64.hip", "p3_kernel_mixed.hip", "p3x_api.hip", "p3x_server.hip",
            "p3x_process.hip", "p3_tables.cpp"]
 
+# Extra flags per translation unit.  The mixed-precision form may use the
+# hardware-backed FP32 division, square root and transcendental sequences
+# (a few ulp instead of correctly rounded); FP64 arithmetic is not affected by
+# these flags, and the FP64 form is a translation unit of its own.
+FILE_FLAGS: dict[str, list[str]] = {
+    "p3_kernel_mixed.hip": ["-fno-hip-fp32-correctly-rounded-divide-sqrt", "-fapprox-func"],
+}
+
 
 def _compile(src: str) -> str:
     out = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC] + COMMON + ["-c", os.path.join(CSRC, src), "-o", out]
+    cmd = [HIPCC] + COMMON + FILE_FLAGS.get(src, []) + ["-c", os.path.join(CSRC, src), "-o", out]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed on {src}:\n{r.stderr[-4000:]}")

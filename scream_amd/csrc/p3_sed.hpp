@@ -112,7 +112,7 @@ __device__ __forceinline__ void sediment(const KParams& p, int i, int kbase, con
       V2[t] = 0;
       if (k >= k_top && k <= k_bot) {  // k_bot < nlev, so k is a real level
         const double rho = g.rho[t];
-        if (SP == 0) {
+        if constexpr (SP == 0) {
           // cloud_sedimentation, P3I/p3_cloud_sed_impl.hpp:85-112
           // in-cloud values in the arithmetic type of the form (the state stays FP64)
           const R cfl = R(cf[t]);
@@ -135,7 +135,7 @@ __device__ __forceinline__ void sediment(const KParams& p, int i, int kbase, con
             V1[t] = double(acn * m_tgamma(R(4 + bcn) + mu_c) * dum / m_tgamma(mu_c + R(4)));
             if (p.predictNc) V2[t] = double(acn * m_tgamma(R(1 + bcn) + mu_c) * dum / m_tgamma(mu_c + R(1)));
           }
-        } else if (SP == 1) {
+        } else if constexpr (SP == 1) {
           // rain_sedimentation + compute_rain_fall_velocity, P3I/p3_rain_sed_impl.hpp:14-34,104-126
           const R cfr = R(cf[t]);
           const R qr = R(q[0][t]), nr = R(q[1][t]);

@@ -22,15 +22,16 @@ namespace p3x_detail {
 int dev_fail(hipError_t e) { return e == hipErrorOutOfMemory ? P3X_ERR_NO_MEMORY : P3X_ERR_DEVICE; }
 
 // Default launch shape per (precision, nlev), from sweeps on MI355X at 21600x72
-// and 16384x128 in both precisions: 64 columns x 256 threads for part 1 + 2
-// (one wavefront per SIMD: the two-wave variant's spills cost more than the
-// occupancy gains), and 32 sedimentation lanes per column up to 160 levels
-// (at most 5 levels per lane; more push the kernel to one wavefront per SIMD).
+// and 16384x128 in both precisions.  Part 1 + 2: 64 columns x 256 threads at one
+// wavefront per SIMD in FP64 (forcing two costs more in spills than the
+// occupancy gains), 32 columns x 256 threads at two wavefronts per SIMD in the
+// mixed form.  Sedimentation: 32 lanes per column up to 160 levels (at most 5
+// levels per lane; more push the kernel to one wavefront per SIMD).
 void default_shape(int precision, int nk, p3d::Shape* sh) {
-  (void)precision;
-  sh->cb = 64;
+  const bool mixed = precision == P3X_MIXED;
+  sh->cb = mixed ? 32 : 64;
   sh->nt = 256;
-  sh->wpe = 1;
+  sh->wpe = mixed ? 2 : 1;
   sh->lpc = nk <= 48 ? 16 : (nk <= 32 * 5 ? 32 : 64);
 }
 
@@ -259,13 +260,11 @@ extern "C" int p3x_main(p3x_context* c, const p3x_runtime* runtime, const p3x_pr
   if (o.precision != P3X_FP64 && o.precision != P3X_MIXED) return P3X_ERR_ARG;
   if (nk > 64 * p3d::kMaxLevelsPerLane) return P3X_ERR_ARG;
   p3d::Shape sh;
-  default_shape(o.precision, nk, &sh);
-  if (o.columns_per_block != 0 || o.threads_per_block != 0) {
-    sh.cb = o.columns_per_block;
-    sh.nt = o.threads_per_block;
+  {
+    const int st = resolve_shape(o.precision, nk, o.columns_per_block, o.threads_per_block,
+                                 o.sed_lanes_per_column, o.part12_waves_per_simd, &sh);
+    if (st != P3X_OK) return st;
   }
-  if (o.sed_lanes_per_column != 0) sh.lpc = o.sed_lanes_per_column;
-  if (o.part12_waves_per_simd != 0) sh.wpe = o.part12_waves_per_simd;
   hipStream_t s = (hipStream_t)o.stream;
   if (nj == 0) return P3X_OK;
   if (!fields_present(ps, di, dout, hist)) return P3X_ERR_ARG;
@@ -308,9 +307,10 @@ namespace p3x_detail {
 
 int resolve_shape(int precision, int nk_max, int cb, int nt, int lpc, int wpe, p3d::Shape* sh) {
   default_shape(precision, nk_max, sh);
-  if (cb != 0 || nt != 0) {
+  if (cb != 0 || nt != 0) {  // an explicit block shape runs at one wavefront per SIMD unless told otherwise
     sh->cb = cb;
     sh->nt = nt;
+    sh->wpe = 1;
   }
   if (lpc != 0) sh->lpc = lpc;
   if (wpe != 0) sh->wpe = wpe;
