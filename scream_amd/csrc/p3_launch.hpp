@@ -35,8 +35,9 @@ inline int sed_levels_per_lane(int nlev, int lpc) {
 }
 
 // hipErrorInvalidValue for a shape that is not compiled in.
-hipError_t launch_p3_fp64(const KParams& p, const Shape& sh, hipStream_t s);
-hipError_t launch_p3_mixed(const KParams& p, const Shape& sh, hipStream_t s);
+// `between` (optional) is recorded on s between the two kernels.
+hipError_t launch_p3_fp64(const KParams& p, const Shape& sh, hipStream_t s, hipEvent_t between = nullptr);
+hipError_t launch_p3_mixed(const KParams& p, const Shape& sh, hipStream_t s, hipEvent_t between = nullptr);
 
 // ---- batched form (p3_batch.hpp)
 // Sets whose sedimentation kernel uses the same levels-per-lane count.

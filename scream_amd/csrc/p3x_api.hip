@@ -210,6 +210,7 @@ extern "C" int p3x_init(const p3x_table_data* t, p3x_context** out) {
   if (e == hipSuccess) e = hipMemset(c->d_err, 0, sizeof(unsigned));
   if (e == hipSuccess) e = hipEventCreate(&c->ev0);
   if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+  if (e == hipSuccess) e = hipEventCreate(&c->ev_mid);
   if (e == hipSuccess) e = hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, c->device);
   if (e != hipSuccess) {
     p3x_finalize(c);
@@ -229,6 +230,7 @@ extern "C" void p3x_finalize(p3x_context* c) {
   (void)hipFree(c->d_err);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->ev_mid) (void)hipEventDestroy(c->ev_mid);
   delete c;
 }
 
@@ -286,7 +288,7 @@ extern "C" int p3x_main(p3x_context* c, const p3x_runtime* runtime, const p3x_pr
 
   hipError_t e = hipEventRecord(c->ev0, s);
   if (e != hipSuccess) return P3X_ERR_DEVICE;
-  e = o.precision == P3X_FP64 ? p3d::launch_p3_fp64(k, sh, s) : p3d::launch_p3_mixed(k, sh, s);
+  e = o.precision == P3X_FP64 ? p3d::launch_p3_fp64(k, sh, s, c->ev_mid) : p3d::launch_p3_mixed(k, sh, s, c->ev_mid);
   if (e == hipErrorInvalidValue) return P3X_ERR_ARG;
   if (e != hipSuccess) return P3X_ERR_DEVICE;
   e = hipEventRecord(c->ev1, s);
@@ -299,6 +301,17 @@ extern "C" int p3x_main(p3x_context* c, const p3x_runtime* runtime, const p3x_pr
     if (hipEventElapsedTime(&ms, c->ev0, c->ev1) == hipSuccess) *elapsed_us = 1000.0 * ms;
   }
   return st;
+}
+
+extern "C" int p3x_last_kernel_times(p3x_context* c, double* part12_us, double* sed_us) {
+  if (!c) return P3X_ERR_ARG;
+  if (hipEventSynchronize(c->ev1) != hipSuccess) return P3X_ERR_DEVICE;
+  float a = 0, b = 0;
+  if (hipEventElapsedTime(&a, c->ev0, c->ev_mid) != hipSuccess) return P3X_ERR_DEVICE;
+  if (hipEventElapsedTime(&b, c->ev_mid, c->ev1) != hipSuccess) return P3X_ERR_DEVICE;
+  if (part12_us) *part12_us = 1000.0 * a;
+  if (sed_us) *sed_us = 1000.0 * b;
+  return P3X_OK;
 }
 
 // ============================================================ batched form

@@ -19,6 +19,7 @@ struct p3x_context {
   size_t scratch_bytes = 0;
   unsigned* d_err = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t ev_mid = nullptr;  // between the part 1 + 2 and the sedimentation kernels
   hipStream_t last_stream = nullptr;
   int pending_timing = 0;
 };

@@ -37,7 +37,8 @@ ALL_FIELDS = STATE + INPUTS + OUTS + HIST
 
 # Every symbol include/p3x.h declares.
 EXPORTS = ("p3x_runtime_defaults", "p3x_launch_defaults", "p3x_tables_load", "p3x_init",
-           "p3x_finalize", "p3x_main", "p3x_sync", "p3x_strerror", "p3x_batch_defaults",
+           "p3x_finalize", "p3x_main", "p3x_sync", "p3x_last_kernel_times", "p3x_strerror",
+           "p3x_batch_defaults",
            "p3x_batch_create", "p3x_batch_run", "p3x_batch_tune", "p3x_batch_destroy",
            "p3x_server_defaults", "p3x_server_create", "p3x_server_add_member", "p3x_server_submit",
            "p3x_server_poll", "p3x_server_drain", "p3x_server_stats_get", "p3x_server_destroy",
@@ -181,6 +182,8 @@ def load() -> C.CDLL:
         lib.p3x_main.restype = C.c_int
         lib.p3x_sync.argtypes = [C.c_void_p]
         lib.p3x_sync.restype = C.c_int
+        lib.p3x_last_kernel_times.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        lib.p3x_last_kernel_times.restype = C.c_int
         lib.p3x_strerror.argtypes = [C.c_int]
         lib.p3x_strerror.restype = C.c_char_p
         lib.p3x_batch_defaults.argtypes = [C.POINTER(BatchOptions)]
@@ -311,6 +314,14 @@ class Context:
 
     def sync(self) -> int:
         return self._lib.p3x_sync(self.handle)
+
+    def last_kernel_times(self) -> tuple[float, float]:
+        """Device microseconds of the last main()'s two kernels: (part 1 + 2, sedimentation + part 3)."""
+        a, b = C.c_double(), C.c_double()
+        st = self._lib.p3x_last_kernel_times(self.handle, C.byref(a), C.byref(b))
+        if st != P3X_OK:
+            raise RuntimeError(f"p3x_last_kernel_times: {strerror(st)}")
+        return a.value, b.value
 
 
 def batch_defaults(**overrides) -> BatchOptions:
