@@ -343,6 +343,20 @@ int p3x_server_drain(p3x_server* server);
 int p3x_server_stats_get(p3x_server* server, p3x_server_stats* stats);
 void p3x_server_destroy(p3x_server* server);
 
+/* ---- the ice lookup-table generator: the computation of
+ * components/eam/tools/create_p3_lookupTable/create_p3_lookupTable_1.f90-v4.1
+ * (the program that writes p3_lookup_table_1.dat, :170-1370) on the current HIP
+ * device, and its write statements (:1374-1427).  Cells are ordered (rime
+ * density 5, rime fraction 4, normalized ice mass 50); every value follows the
+ * program's formula and summation order. ---- */
+/* ice_rows[1000][15]: qsave nsave uns ums nagg nrwat vdep eff nlarge nsmall lsave
+ * refl vdep1 dmm rhomm; coll_rows[1000][30][6]: qsave nsave lamrs nrrain qrrain
+ * qsrain (host arrays).  elapsed_ms (optional): wall time of the generation. */
+int p3x_ice_table_generate(double* ice_rows, double* coll_rows, void* stream, double* elapsed_ms);
+/* The rows as a table file p3x_tables_load reads ("VERSION 4.1.1" header, the
+ * program's e15.5 rows). */
+int p3x_ice_table_write(const char* path, const double* ice_rows, const double* coll_rows);
+
 const char* p3x_strerror(int status);
 
 #ifdef __cplusplus

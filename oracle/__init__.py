@@ -27,6 +27,33 @@ ORACLE_PMATH_SO = os.path.join(HERE, "libp3oracle_pmath.so")
 #              uses, for the bit-for-bit comparison with the GPU.
 FLAVOURS = {"glibc": ORACLE_SO, "fortran": ORACLE_FMATH_SO, "portable": ORACLE_PMATH_SO}
 REF_SO = os.path.join(HERE, "_ref", "libp3ref.so")
+TABLEGEN_SO = {"glibc": os.path.join(HERE, "libp3tablegen.so"),
+               "portable": os.path.join(HERE, "libp3tablegen_pmath.so")}
+_tablegen = {}
+
+
+def table_gen(math="glibc", cells=None, nthreads=0):
+    """The ice lookup-table generator restated (p3_table_gen.c).  cells: iterable of
+    flat cell indices ((rime density * 4 + rime fraction) * 50 + mass index), None =
+    all 1000.  Returns (ice [1000, 15], coll [1000, 30, 6]); cells not computed are 0."""
+    so = TABLEGEN_SO[math]
+    if so not in _tablegen:
+        if not os.path.exists(so):
+            raise RuntimeError(f"{so} is not built (run `make -C oracle`)")
+        lib = C.CDLL(so)
+        lib.p3o_table_gen.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
+        lib.p3o_table_gen.restype = None
+        _tablegen[so] = lib
+    lib = _tablegen[so]
+    ice = np.zeros((1000, 15))
+    coll = np.zeros((1000, 30, 6))
+    mask = None
+    if cells is not None:
+        mask = np.zeros(1000, np.uint8)
+        mask[list(cells)] = 1
+    nt = nthreads if nthreads > 0 else min(16, os.cpu_count() or 1)
+    lib.p3o_table_gen(None if mask is None else mask.ctypes.data, ice.ctypes.data, coll.ctypes.data, nt)
+    return ice, coll
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)

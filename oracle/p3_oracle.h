@@ -149,6 +149,11 @@ void p3o_upwind_step(int nk, int k_bot, int k_top, double dt_sub, int nfield,
                      const double* rho, const double* inv_rho, const double* inv_dz,
                      double* const* flux, const double* const* V, double* const* r);
 
+/* The ice lookup-table generator (oracle/p3_table_gen.c, its own library):
+ * cells (i_rhor, i_Fr, i_Qnorm) with mask[cell] set (mask NULL = all 1000);
+ * ice[cell][15] and coll[cell][30][6] as the generator's output rows. */
+void p3o_table_gen(const unsigned char* mask, double* ice, double* coll, int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

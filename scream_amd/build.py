@@ -24,7 +24,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", f"--offload-arch={A
           f"-I{os.path.join(HERE, '..', 'include')}"]
 
 SOURCES = ["p3_kernel_fp64.hip", "p3_kernel_mixed.hip", "p3x_api.hip", "p3x_server.hip",
-           "p3x_process.hip", "p3_tables.cpp"]
+           "p3x_process.hip", "p3_tables.cpp", "p3_table_gen.hip"]
 
 # Extra flags per translation unit.  The mixed-precision form may use the
 # hardware-backed FP32 division, square root and transcendental sequences

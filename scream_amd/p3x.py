@@ -43,7 +43,7 @@ EXPORTS = ("p3x_runtime_defaults", "p3x_launch_defaults", "p3x_tables_load", "p3
            "p3x_server_defaults", "p3x_server_create", "p3x_server_add_member", "p3x_server_submit",
            "p3x_server_poll", "p3x_server_drain", "p3x_server_stats_get", "p3x_server_destroy",
            "p3x_process_create", "p3x_process_run", "p3x_process_destroy", "p3x_column_budget",
-           "p3x_conservation_errors")
+           "p3x_conservation_errors", "p3x_ice_table_generate", "p3x_ice_table_write")
 
 # p3x_atm_fields, in struct order (the P3Microphysics field names)
 ATM_FIELDS = ("p_mid", "p_dry_mid", "pseudo_density", "pseudo_density_dry", "cldfrac_tot",
@@ -172,6 +172,10 @@ def load() -> C.CDLL:
         lib.p3x_launch_defaults.argtypes = [C.POINTER(Launch)]
         lib.p3x_tables_load.argtypes = [C.c_char_p, C.POINTER(TableData)]
         lib.p3x_tables_load.restype = C.c_int
+        lib.p3x_ice_table_generate.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_double)]
+        lib.p3x_ice_table_generate.restype = C.c_int
+        lib.p3x_ice_table_write.argtypes = [C.c_char_p, C.c_void_p, C.c_void_p]
+        lib.p3x_ice_table_write.restype = C.c_int
         lib.p3x_init.argtypes = [C.POINTER(TableData), C.POINTER(C.c_void_p)]
         lib.p3x_init.restype = C.c_int
         lib.p3x_finalize.argtypes = [C.c_void_p]
@@ -246,6 +250,35 @@ def tables_load(path: str = ICE_TABLE) -> TableData:
     if st != P3X_OK:
         raise P3Error(st, f"p3x_tables_load({path})")
     return t
+
+
+# Cells of the ice lookup-table generator: (rime density 5, rime fraction 4, mass 50)
+ICE_TABLE_CELLS, ICE_ROW_COLS, COLL_RAIN_SIZES, COLL_ROW_COLS = 1000, 15, 30, 6
+
+
+def ice_table_generate(stream=None):
+    """Runs the ice lookup-table generator (create_p3_lookupTable_1.f90-v4.1) on the
+    current device.  Returns (ice_rows [1000, 15], coll_rows [1000, 30, 6], wall ms)."""
+    import numpy as np
+    ice = np.zeros((ICE_TABLE_CELLS, ICE_ROW_COLS))
+    coll = np.zeros((ICE_TABLE_CELLS, COLL_RAIN_SIZES, COLL_ROW_COLS))
+    ms = C.c_double()
+    st = load().p3x_ice_table_generate(ice.ctypes.data, coll.ctypes.data, stream, C.byref(ms))
+    if st != P3X_OK:
+        raise P3Error(st, "p3x_ice_table_generate")
+    return ice, coll, ms.value
+
+
+def ice_table_write(path: str, ice_rows, coll_rows) -> None:
+    """Writes generator rows as a table file that tables_load reads."""
+    import numpy as np
+    ice = np.ascontiguousarray(ice_rows, dtype=np.float64)
+    coll = np.ascontiguousarray(coll_rows, dtype=np.float64)
+    if ice.shape != (ICE_TABLE_CELLS, ICE_ROW_COLS) or coll.shape != (ICE_TABLE_CELLS, COLL_RAIN_SIZES, COLL_ROW_COLS):
+        raise ValueError("ice_table_write: rows of the wrong shape")
+    st = load().p3x_ice_table_write(path.encode(), ice.ctypes.data, coll.ctypes.data)
+    if st != P3X_OK:
+        raise P3Error(st, f"p3x_ice_table_write({path})")
 
 
 def _addr(x) -> int:
