@@ -45,6 +45,8 @@ def parse():
                     help="column sets of the batched-form leg; 0 skips it")
     ap.add_argument("--serve-seconds", type=float, default=5.0,
                     help="length of the served-form run (Poisson arrivals); 0 skips it")
+    ap.add_argument("--no-table-gen", action="store_true",
+                    help="skip timing the ice lookup-table generator")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (HBM counter calibration)")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
@@ -295,6 +297,16 @@ def cpu_baseline():
 COPY_PROBE_BYTES = 1 << 30
 
 
+def table_generator(reps=3):
+    """Wall milliseconds of the device ice lookup-table generator (the whole
+    5 x 4 x 50-cell table of create_p3_lookupTable_1, about 3e10 inner iterations
+    when run in sequence), median of reps after one untimed run."""
+    from scream_amd import p3x
+    p3x.ice_table_generate()
+    ms = sorted(p3x.ice_table_generate()[2] for _ in range(reps))
+    return {"ms": ms[len(ms) // 2], "cells": 1000, "reps": reps}
+
+
 def copy_probe():
     """A plain streaming copy of 1 GiB as one elementwise kernel (dst = src * 1), three
     times: under rocprofv3 --pmc its FETCH_SIZE / WRITE_SIZE per dispatch calibrate the
@@ -437,6 +449,8 @@ def main():
         line["batched"] = batched(ctx, args.batch_sets, max(5, args.steps // 2))
     if args.serve_seconds > 0:
         line["served"] = served(ctx, args.serve_seconds)
+    if not args.no_table_gen:
+        line["table_generator"] = table_generator()
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline()
     ctx.close()
