@@ -137,15 +137,15 @@ enum { P3X_FP64 = 0, P3X_MIXED = 1 };
 typedef struct p3x_launch {
   int precision;           /* P3X_FP64 (drop-in) or P3X_MIXED (FP32 bulk rates)  */
   int ld;                  /* row stride of the level-major arrays; 0 means nj    */
-  /* part 1 + part 2 kernel: columns per block x threads per block, both 0 for
-   * the default, else one of 16x128, 16x256, 32x256, 64x256, 32x512, 64x512 */
+  /* part 1 kernel: columns per block x threads per block, both 0 for the
+   * default, else one of 16x128, 16x256, 32x256, 64x256, 32x512, 64x512      */
   int columns_per_block;
   int threads_per_block;
   /* sedimentation + part 3 kernel: lanes per column, 0 for the default, else
    * 16, 32 or 64; nk may not exceed 8 x this (nk <= 512 in any case)         */
   int sed_lanes_per_column;
-  /* part 1 + part 2 kernel: wavefronts per SIMD its registers must allow, 0 for
-   * the default (1), else 1 or 2 (2: 16x256, 32x256, 64x256 only)            */
+  /* part 2 kernel: wavefronts per SIMD its registers must allow, 0 for the
+   * default (2), else 1 or 2; an explicit block shape without it means 1    */
   int part12_waves_per_simd;
   void* stream;            /* hipStream_t; NULL = default stream                  */
   int async;               /* 1: return after enqueueing (status from p3x_sync)   */
@@ -213,7 +213,7 @@ typedef struct p3x_column_set {
 
 typedef struct p3x_batch_options {
   int precision;             /* P3X_FP64 or P3X_MIXED                                     */
-  int columns_per_block;     /* part 1 + part 2 shape, as in p3x_launch (0 = default)    */
+  int columns_per_block;     /* part 1 shape, as in p3x_launch (0 = default)              */
   int threads_per_block;
   int sed_lanes_per_column;  /* 0 = default                                               */
   int part12_waves_per_simd; /* 0 = default, 1 or 2, as in p3x_launch                    */

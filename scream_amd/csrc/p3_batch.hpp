@@ -26,15 +26,28 @@ __device__ __forceinline__ int find_set(const int* __restrict__ first, int nsets
   return lo;
 }
 
-template <class R, int CB, int NT, int WPE>
-__global__ __launch_bounds__(NT, WPE) void p3_part12_batch(const KParams* __restrict__ sets,
-                                                      const int* __restrict__ first, int nsets) {
+template <class R, int CB, int NT>
+__global__ __launch_bounds__(NT) void p3_part1_batch(const KParams* __restrict__ sets,
+                                                     const int* __restrict__ first, int nsets) {
   __shared__ int s_i[NT];
   const auto sets4 = (const __attribute__((address_space(4))) KParams*)sets;
   const int total = first[nsets];
   for (int vb = blockIdx.x; vb < total; vb += gridDim.x) {
     const int s = find_set(first, nsets, vb);
-    part12_block<R, CB, NT>(*(const KParams*)(sets4 + s), vb - first[s], s_i);
+    part1_block<R, CB, NT>(*(const KParams*)(sets4 + s), vb - first[s], s_i);
+  }
+}
+
+// first: the part 2 tile offsets (part2_tiles per set).
+template <class R, int WPE>
+__global__ __launch_bounds__(kPart2Cols * kPart2Levels, WPE) void p3_part2_batch(const KParams* __restrict__ sets,
+                                                                                const int* __restrict__ first,
+                                                                                int nsets) {
+  const auto sets4 = (const __attribute__((address_space(4))) KParams*)sets;
+  const int total = first[nsets];
+  for (int vb = blockIdx.x; vb < total; vb += gridDim.x) {
+    const int s = find_set(first, nsets, vb);
+    part2_tile<R>(*(const KParams*)(sets4 + s), vb - first[s]);
   }
 }
 

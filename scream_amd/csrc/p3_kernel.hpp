@@ -1,15 +1,22 @@
-// p3_kernel.hpp -- first kernel of a P3 step: p3_main_init, part 1 and part 2.
+// p3_kernel.hpp -- p3_main_init, part 1 and part 2 of a P3 step (the first
+// two of its three kernels; sedimentation and part 3 are in p3_sed.hpp).
 //
-// Work decomposition: a workgroup owns CB consecutive columns; its NT =
+// Part 1 (p3_part1_kernel): a workgroup owns CB consecutive columns; its NT =
 // 64*waves threads split every column's levels into NCH = NT/CB contiguous
 // chunks; thread t handles column c = t % CB (so CB consecutive lanes read CB
-// consecutive doubles of a level-major row: one or more full cache lines) and
-// chunk t / CB.  Parts 1 and 2 are pointwise on the thread's own levels; the
-// two per-column early-exit flags are OR-reductions through LDS.  Between the
-// parts a level is only touched by its owning thread, so the level-major
-// scratch in HBM needs no synchronisation beyond program order.  The column's
-// status (skipped after part 1, after part 2, or a full step) goes to the
-// sedimentation kernel (p3_sed.hpp) through KParams::colstat.
+// consecutive doubles of a level-major row) and chunk t / CB.  The column's
+// early-exit flag (nucleationPossible || hydrometeorsPresent) is an
+// OR-reduction through LDS; the column status goes to KParams::colstat.
+//
+// Part 2 (p3_part2_kernel): pointwise in the level once the column's part-1
+// status is known, so one thread per (level, column): a workgroup is 64
+// consecutive columns x 4 levels, one level per wavefront.  Every wavefront
+// then carries the same work whatever the column count, and a grid of ncol/64
+// x nlev/4 blocks keeps every CU busy to the end (one block of whole columns
+// per CU left most of the chip idle in a second, partial round).  A level
+// with hydrometeors after part 2 ORs bit 2 into its column's status, so the
+// sedimentation kernel sees 0 (skipped after part 1), 1 (after part 2) or 3
+// (full step).
 //
 // The step follows components/eamxx/src/physics/p3/impl/p3_main_impl.hpp:74-260.
 #pragma once
@@ -35,7 +42,7 @@ struct KParams {
   // P3HistoryOnly (in/out unless exch_zero)
   double *liq_ice_exchange, *vap_liq_exchange, *vap_ice_exchange;
   double* scratch;      // SC_COUNT x nlev x ld
-  int* colstat;         // per column: 0 skipped after part 1, 1 after part 2, 2 full step
+  int* colstat;         // per column: 0 skipped after part 1, 1 after part 2, 3 full step
   unsigned* err;        // device error word
   int *dbg_status, *dbg_substeps, *dbg_ice_idx;
   Tables tab;
@@ -43,14 +50,17 @@ struct KParams {
   int ncol, nlev, ld;   // ld: row stride (elements) of every level-major array
   double dt;
   int predictNc, prescribedCCN, exch_zero;
+  double e_melt;        // saturation vapour pressure over liquid at Tmelt (qv_sat_dry(Tmelt, p) numerator)
 };
 
 constexpr int kMaxSubsteps = 100000;
+// Column status bits: part 1 found work, part 2 left hydrometeors.
+constexpr int kStatusPart1 = 1, kStatusPart2 = 2, kStatusFull = 3;
 
 // One block's work: columns bid*CB .. bid*CB+CB-1 of the column set p.
 // s_i is an LDS array of NT ints.
 template <class R, int CB, int NT>
-__device__ __forceinline__ void part12_block(const KParams& p, int bid, int* s_i) {
+__device__ __forceinline__ void part1_block(const KParams& p, int bid, int* s_i) {
   constexpr int NCH = NT / CB;
   static_assert(NT % CB == 0, "block must hold whole chunk rows");
 
@@ -152,77 +162,90 @@ __device__ __forceinline__ void part12_block(const KParams& p, int bid, int* s_i
     }
   }
   const int cflags = red_or(flags);
-  // status: 0 skipped after part 1, 1 skipped after part 2, 2 full step
-  int status = (col_ok && (cflags & 3)) ? 1 : 0;
+  if (col_ok && ch == 0) p.colstat[i] = (cflags & 3) ? kStatusPart1 : 0;
+  if (err) atomicOr(p.err, err);
+}
 
-  // ================= p3_main_part2 (P3I/p3_main_impl_part2.hpp:113-524)
-  // saturation vapour pressure over liquid at Tmelt (melting and wet growth use qv_sat_dry(Tmelt, p))
-  const double e_melt = mk_svp(Tmelt, false, err);
-  int hydro = 0;
-  if (col_ok) {
-    for (int k = k0; k < k1; ++k) {
-      const size_t x = at(k);
-      bool done = false;
-      if (status == 1) {
-        Lvl s;
-        s.qc = p.qc[x]; s.qr = p.qr[x]; s.qi = p.qi[x];
-        s.T_atm = scr(SC_T, k);
-        s.qv_supersat_i = scr(SC_SUPI, k);
-        const bool skip_all = s.qc < QSMALL && s.qr < QSMALL && s.qi < QSMALL && s.T_atm < T_zerodegc &&
-                              s.qv_supersat_i < -0.05;
-        if (!skip_all) {
-          s.nc = p.nc[x]; s.nr = p.nr[x]; s.qm = p.qm[x]; s.ni = p.ni[x]; s.bm = p.bm[x];
-          s.qv = p.qv[x]; s.th = p.th[x];
-          s.pres = p.pres[x];
-          s.rho = p.dpres[x] / p.dz[x] / gravit;
-          s.inv_rho = 1 / s.rho;
-          s.qv_sat_l = scr(SC_QVSL, k);
-          s.qv_sat_i = scr(SC_QVSI, k);
-          s.rhofaci = scr(SC_RHOFACI, k);
-          s.acn = scr(SC_ACN, k);
-          s.cfl = p.cld_frac_l[x]; s.cfr = p.cld_frac_r[x]; s.cfi = p.cld_frac_i[x];
-          s.icfl = 1 / s.cfl; s.icfr = 1 / s.cfr; s.icfi = 1 / s.cfi;
-          s.inv_exner = p.inv_exner[x];
-          s.qv_prev = p.qv_prev[x]; s.t_prev = p.t_prev[x];
-          s.ni_activated = p.ni_activated[x];
-          if (part2_level<R>(rt, p.tab, dt, p.predictNc, p.prescribedCCN, e_melt, s, err)) hydro = 1;
-          p.qc[x] = s.qc; p.nc[x] = s.nc; p.qr[x] = s.qr; p.nr[x] = s.nr; p.qi[x] = s.qi;
-          p.ni[x] = s.ni; p.qm[x] = s.qm; p.bm[x] = s.bm; p.qv[x] = s.qv; p.th[x] = s.th;
-          p.qv2qi_depos_tend[x] = s.qv2qi_depos_tend;
-          p.precip_total_tend[x] = s.precip_total_tend;
-          p.nevapr[x] = s.nevapr;
-          p.vap_ice_exchange[x] = s.vap_ice_exchange;
-          p.vap_liq_exchange[x] = s.vap_liq_exchange;
-          p.liq_ice_exchange[x] = s.liq_ice_exchange;
-          done = true;
-        }
-      }
-      if (!done) {
-        p.qv2qi_depos_tend[x] = 0;
-        p.precip_total_tend[x] = 0;
-        p.nevapr[x] = 0;
-        if (p.exch_zero) {
-          p.vap_ice_exchange[x] = 0;
-          p.vap_liq_exchange[x] = 0;
-          p.liq_ice_exchange[x] = 0;
-        }
-      }
+template <class R, int CB, int NT>
+__global__ __launch_bounds__(NT) void p3_part1_kernel(const KParams p) {
+  __shared__ int s_i[NT];
+  part1_block<R, CB, NT>(p, blockIdx.x, s_i);
+}
+
+// Part 2 tiles of a column set: 64 columns x 4 levels each, columns fastest.
+constexpr int kPart2Cols = 64, kPart2Levels = 4;
+__host__ __device__ __forceinline__ int part2_tiles(int ncol, int nlev) {
+  return ((ncol + kPart2Cols - 1) / kPart2Cols) * ((nlev + kPart2Levels - 1) / kPart2Levels);
+}
+
+// ================= p3_main_part2 (P3I/p3_main_impl_part2.hpp:113-524) for the
+// level of this wavefront and the column of this lane in tile `tile`.
+template <class R>
+__device__ __forceinline__ void part2_tile(const KParams& p, int tile) {
+  const int ntc = (p.ncol + kPart2Cols - 1) / kPart2Cols;
+  const int i = (tile % ntc) * kPart2Cols + (threadIdx.x & 63);
+  const int k = (tile / ntc) * kPart2Levels + (threadIdx.x >> 6);
+  if (i >= p.ncol || k >= p.nlev) return;
+  const int nlev = p.nlev;
+  const size_t ld = (size_t)p.ld;
+  const size_t x = (size_t)k * ld + i;
+  auto scr = [&](int f) -> double { return p.scratch[((size_t)f * nlev + k) * ld + i]; };
+  unsigned err = 0;
+  bool done = false;
+  // bit 1 only: other levels of the column may already have set bit 2
+  if (p.colstat[i] & kStatusPart1) {
+    Lvl s;
+    s.qc = p.qc[x]; s.qr = p.qr[x]; s.qi = p.qi[x];
+    s.T_atm = scr(SC_T);
+    s.qv_supersat_i = scr(SC_SUPI);
+    const bool skip_all = s.qc < QSMALL && s.qr < QSMALL && s.qi < QSMALL && s.T_atm < T_zerodegc &&
+                          s.qv_supersat_i < -0.05;
+    if (!skip_all) {
+      s.nc = p.nc[x]; s.nr = p.nr[x]; s.qm = p.qm[x]; s.ni = p.ni[x]; s.bm = p.bm[x];
+      s.qv = p.qv[x]; s.th = p.th[x];
+      s.pres = p.pres[x];
+      s.rho = p.dpres[x] / p.dz[x] / gravit;
+      s.inv_rho = 1 / s.rho;
+      s.qv_sat_l = scr(SC_QVSL);
+      s.qv_sat_i = scr(SC_QVSI);
+      s.rhofaci = scr(SC_RHOFACI);
+      s.acn = scr(SC_ACN);
+      s.cfl = p.cld_frac_l[x]; s.cfr = p.cld_frac_r[x]; s.cfi = p.cld_frac_i[x];
+      s.icfl = 1 / s.cfl; s.icfr = 1 / s.cfr; s.icfi = 1 / s.cfi;
+      s.inv_exner = p.inv_exner[x];
+      s.qv_prev = p.qv_prev[x]; s.t_prev = p.t_prev[x];
+      s.ni_activated = p.ni_activated[x];
+      // hydrometeorsPresent after part 2 (P3I/p3_main_impl.hpp:255-260): a column flag
+      if (part2_level<R>(p.rt, p.tab, p.dt, p.predictNc, p.prescribedCCN, p.e_melt, s, err))
+        atomicOr(&p.colstat[i], kStatusPart2);
+      p.qc[x] = s.qc; p.nc[x] = s.nc; p.qr[x] = s.qr; p.nr[x] = s.nr; p.qi[x] = s.qi;
+      p.ni[x] = s.ni; p.qm[x] = s.qm; p.bm[x] = s.bm; p.qv[x] = s.qv; p.th[x] = s.th;
+      p.qv2qi_depos_tend[x] = s.qv2qi_depos_tend;
+      p.precip_total_tend[x] = s.precip_total_tend;
+      p.nevapr[x] = s.nevapr;
+      p.vap_ice_exchange[x] = s.vap_ice_exchange;
+      p.vap_liq_exchange[x] = s.vap_liq_exchange;
+      p.liq_ice_exchange[x] = s.liq_ice_exchange;
+      done = true;
     }
   }
-  if (red_or(hydro) && status == 1) status = 2;
-
-  if (col_ok && ch == 0) {
-    p.colstat[i] = status;
-    if (p.dbg_status) p.dbg_status[i] = status;
+  if (!done) {
+    p.qv2qi_depos_tend[x] = 0;
+    p.precip_total_tend[x] = 0;
+    p.nevapr[x] = 0;
+    if (p.exch_zero) {
+      p.vap_ice_exchange[x] = 0;
+      p.vap_liq_exchange[x] = 0;
+      p.liq_ice_exchange[x] = 0;
+    }
   }
   if (err) atomicOr(p.err, err);
 }
 
 // WPE: minimum wavefronts per SIMD the register allocation must allow (1 or 2).
-template <class R, int CB, int NT, int WPE>
-__global__ __launch_bounds__(NT, WPE) void p3_part12_kernel(const KParams p) {
-  __shared__ int s_i[NT];
-  part12_block<R, CB, NT>(p, blockIdx.x, s_i);
+template <class R, int WPE>
+__global__ __launch_bounds__(kPart2Cols * kPart2Levels, WPE) void p3_part2_kernel(const KParams p) {
+  part2_tile<R>(p, blockIdx.x);
 }
 
 }  // namespace p3d

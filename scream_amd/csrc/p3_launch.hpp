@@ -7,11 +7,11 @@
 
 namespace p3d {
 
-// Shape of one step's two launches.
-//   part 1 + part 2: cb columns per block x nt threads per block
-//     (compiled: 16x128, 16x256, 32x256, 64x256, 32x512, 64x512), register
-//     allocation for wpe = 1 or 2 wavefronts per SIMD (2 only for 16x256,
-//     32x256 and 64x256: fewer registers, with spills);
+// Shape of one step's three launches.
+//   part 1: cb columns per block x nt threads per block (compiled: 16x128,
+//     16x256, 32x256, 64x256, 32x512, 64x512);
+//   part 2: 64 columns x 4 levels per block, register allocation for wpe = 1
+//     or 2 wavefronts per SIMD (2: fewer registers, with spills in FP64);
 //   sedimentation + part 3: lpc lanes per column (16, 32 or 64), each lane
 //     holding L = ceil(nlev / lpc) levels (at most 8), 4 wavefronts per block.
 struct Shape {
@@ -35,7 +35,7 @@ inline int sed_levels_per_lane(int nlev, int lpc) {
 }
 
 // hipErrorInvalidValue for a shape that is not compiled in.
-// `between` (optional) is recorded on s between the two kernels.
+// `between` (optional) is recorded on s after part 2, before sedimentation.
 hipError_t launch_p3_fp64(const KParams& p, const Shape& sh, hipStream_t s, hipEvent_t between = nullptr);
 hipError_t launch_p3_mixed(const KParams& p, const Shape& sh, hipStream_t s, hipEvent_t between = nullptr);
 
@@ -50,17 +50,18 @@ struct SedClass {
 };
 struct BatchPlan {
   Shape shape;
-  const KParams* sets;  // device array, all sets (part 1 + part 2 launch)
-  const int* first;     // device array, nsets + 1 entries
+  const KParams* sets;  // device array, all sets (part 1 and part 2 launches)
+  const int* first;     // device array, nsets + 1 entries: part 1 blocks
+  const int* first2;    // device array, nsets + 1 entries: part 2 tiles
   int nsets;
-  int grid;             // persistent blocks of the part 1 + part 2 launch
+  int grid, grid2;      // persistent blocks of the part 1 and part 2 launches
   int nclass;
   SedClass cls[kMaxLevelsPerLane];
 };
 hipError_t launch_p3_batch_fp64(const BatchPlan& b, hipStream_t s);
 hipError_t launch_p3_batch_mixed(const BatchPlan& b, hipStream_t s);
 // Resident blocks per CU of the batched kernels for this shape (occupancy API).
-hipError_t occupancy_p3_fp64(const Shape& sh, int L, int* blocks12, int* blocks_sed);
-hipError_t occupancy_p3_mixed(const Shape& sh, int L, int* blocks12, int* blocks_sed);
+hipError_t occupancy_p3_fp64(const Shape& sh, int L, int* blocks1, int* blocks2, int* blocks_sed);
+hipError_t occupancy_p3_mixed(const Shape& sh, int L, int* blocks1, int* blocks2, int* blocks_sed);
 
 }  // namespace p3d

@@ -55,8 +55,8 @@ enum : unsigned { ERR_BAD_TEMPERATURE = 1u, ERR_SUBSTEP_LIMIT = 2u };
 
 // ---------------------------------------------------------------- saturation (FP64 always)
 // MurphyKoop_svp, PSH:38-77 (with check_temperature, PSH:14-36, as a flag)
-__device__ __forceinline__ double mk_svp(double t, bool ice, unsigned& err) {
-  if (t <= 0 || isnan(t)) err |= ERR_BAD_TEMPERATURE;
+__host__ __device__ __forceinline__ double mk_svp(double t, bool ice, unsigned& err) {
+  if (t <= 0 || t != t) err |= ERR_BAD_TEMPERATURE;  // t != t: NaN
   if (ice && t < Tmelt) {
     return p3m_exp(9.550426 - (5723.265 / t) + (3.53068 * p3m_log(t)) - (0.00728332 * t));
   }

@@ -409,7 +409,8 @@ __device__ __forceinline__ void sed_block(const KParams& p, int bid) {
   // Whole groups skip together: the cross-lane operations below stay inside a group.
   if (i < p.ncol) {
     const double inv_dt = 1 / p.dt;
-    const bool active = p.colstat[i] == 2;
+    const int status = p.colstat[i];
+    const bool active = status == kStatusFull;
     double pflux[L];
 #pragma unroll
     for (int t = 0; t < L; ++t) pflux[t] = 0;
@@ -449,6 +450,8 @@ __device__ __forceinline__ void sed_block(const KParams& p, int bid) {
     }
     if (j == 0) {
       p.precip_liq_flux[i] = 0;  // the top interface
+      // 0 skipped after part 1, 1 skipped after part 2, 2 full step
+      if (p.dbg_status) p.dbg_status[i] = status == kStatusFull ? 2 : status;
       if (p.dbg_substeps) {
         p.dbg_substeps[i] = ns_c;
         p.dbg_substeps[(size_t)p.ncol + i] = ns_r;
@@ -460,7 +463,7 @@ __device__ __forceinline__ void sed_block(const KParams& p, int bid) {
   __syncthreads();
   for (int e = threadIdx.x; e < CB * nlev; e += 64 * WPB) {
     const int k = e / CB, c = e % CB, ic = col0 + c;
-    if (ic < p.ncol) part3_level<R>(p, k, ic, p.colstat[ic] == 2);
+    if (ic < p.ncol) part3_level<R>(p, k, ic, p.colstat[ic] == kStatusFull);
   }
   if (err) atomicOr(p.err, err);
 }

@@ -22,16 +22,16 @@ namespace p3x_detail {
 int dev_fail(hipError_t e) { return e == hipErrorOutOfMemory ? P3X_ERR_NO_MEMORY : P3X_ERR_DEVICE; }
 
 // Default launch shape per (precision, nlev), from sweeps on MI355X at 21600x72
-// and 16384x128 in both precisions.  Part 1 + 2: 64 columns x 256 threads at one
-// wavefront per SIMD in FP64 (forcing two costs more in spills than the
-// occupancy gains), 32 columns x 256 threads at two wavefronts per SIMD in the
-// mixed form.  Sedimentation: 32 lanes per column up to 160 levels (at most 5
-// levels per lane; more push the kernel to one wavefront per SIMD).
+// and 16384x128 in both precisions.  Part 1: 32 columns x 256 threads.  Part 2:
+// compiled for two wavefronts per SIMD (in FP64 that spills a little, which
+// costs less than the latency a single wavefront leaves exposed).
+// Sedimentation: 32 lanes per column up to 160 levels (at most 5 levels per
+// lane; more push the kernel to one wavefront per SIMD).
 void default_shape(int precision, int nk, p3d::Shape* sh) {
-  const bool mixed = precision == P3X_MIXED;
-  sh->cb = mixed ? 32 : 64;
+  (void)precision;
+  sh->cb = 32;
   sh->nt = 256;
-  sh->wpe = mixed ? 2 : 1;
+  sh->wpe = 2;
   sh->lpc = nk <= 48 ? 16 : (nk <= 32 * 5 ? 32 : 64);
 }
 
@@ -121,6 +121,10 @@ void fill_params(const p3x_context* c, const p3x_runtime* runtime, const p3x_pro
   k.predictNc = infra->predictNc ? 1 : 0;
   k.prescribedCCN = infra->prescribedCCN ? 1 : 0;
   k.exch_zero = infra->exchange_zeroed_on_entry ? 1 : 0;
+  // qv_sat_dry(Tmelt, p) = ep_2 e_melt / max(p, 1e-3) for melting and wet growth: the same
+  // portable arithmetic on the host as on the device
+  unsigned no_err = 0;
+  k.e_melt = p3d::mk_svp(p3c::Tmelt, false, no_err);
 }
 
 // Every field pointer of a column set is non-null.
@@ -335,7 +339,11 @@ int resolve_shape(int precision, int nk_max, int cb, int nt, int lpc, int wpe, p
 int layout_batch(const p3d::KParams* sets, int n, const p3d::Shape& sh, Layout* out) {
   Layout& L = *out;
   L.first.assign(n + 1, 0);
-  for (int i = 0; i < n; ++i) L.first[i + 1] = L.first[i] + (sets[i].ncol + sh.cb - 1) / sh.cb;
+  L.first2.assign(n + 1, 0);
+  for (int i = 0; i < n; ++i) {
+    L.first[i + 1] = L.first[i] + (sets[i].ncol + sh.cb - 1) / sh.cb;
+    L.first2[i + 1] = L.first2[i] + p3d::part2_tiles(sets[i].ncol, sets[i].nlev);
+  }
   const int cols_sed = p3d::kSedWaves * (64 / sh.lpc);
   std::vector<int> cls_of(n);
   for (int i = 0; i < n; ++i) {
@@ -366,9 +374,9 @@ int grid_for(int ncu, int blocks_per_cu, int total, int resident) {
   return (int)std::max(1L, std::min<long>(g, std::max(total, 1)));
 }
 
-hipError_t occupancy(int precision, const p3d::Shape& sh, int L, int* r12, int* rsed) {
-  return precision == P3X_FP64 ? p3d::occupancy_p3_fp64(sh, L, r12, rsed)
-                               : p3d::occupancy_p3_mixed(sh, L, r12, rsed);
+hipError_t occupancy(int precision, const p3d::Shape& sh, int L, int* r1, int* r2, int* rsed) {
+  return precision == P3X_FP64 ? p3d::occupancy_p3_fp64(sh, L, r1, r2, rsed)
+                               : p3d::occupancy_p3_mixed(sh, L, r1, r2, rsed);
 }
 
 hipError_t launch_plan(int precision, const p3d::BatchPlan& plan, hipStream_t s) {
@@ -382,8 +390,8 @@ struct p3x_batch {
   p3x_batch_options opt{};
   std::vector<p3d::KParams> sets;  // in the caller's order
   double* d_scratch = nullptr;
-  p3d::KParams* d_sets = nullptr;  // caller's order (part 1 + part 2 launch)
-  int* d_first = nullptr;
+  p3d::KParams* d_sets = nullptr;  // caller's order (part 1 and part 2 launches)
+  int* d_first = nullptr;          // part 1 blocks, then part 2 tiles (2 x (nsets + 1))
   p3d::KParams* d_sets_sed = nullptr;  // grouped by sedimentation class
   int* d_first_sed = nullptr;
   p3d::BatchPlan plan{};
@@ -428,8 +436,10 @@ int build_plan(p3x_batch* b) {
   std::vector<p3d::KParams> sed_sets;
   for (int i : lay.sed_order) sed_sets.push_back(b->sets[i]);
 
-  e = hipMalloc((void**)&b->d_first, lay.first.size() * sizeof(int));
-  if (e == hipSuccess) e = hipMemcpy(b->d_first, lay.first.data(), lay.first.size() * sizeof(int), hipMemcpyHostToDevice);
+  std::vector<int> firsts(lay.first);
+  firsts.insert(firsts.end(), lay.first2.begin(), lay.first2.end());
+  e = hipMalloc((void**)&b->d_first, firsts.size() * sizeof(int));
+  if (e == hipSuccess) e = hipMemcpy(b->d_first, firsts.data(), firsts.size() * sizeof(int), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMalloc((void**)&b->d_sets_sed, sed_sets.size() * sizeof(p3d::KParams));
   if (e == hipSuccess)
     e = hipMemcpy(b->d_sets_sed, sed_sets.data(), sed_sets.size() * sizeof(p3d::KParams), hipMemcpyHostToDevice);
@@ -442,6 +452,7 @@ int build_plan(p3x_batch* b) {
   b->plan.shape = sh;
   b->plan.sets = b->d_sets;
   b->plan.first = b->d_first;
+  b->plan.first2 = b->d_first + (n + 1);
   b->plan.nsets = n;
   b->plan.nclass = (int)lay.cls.size();
   for (int ci = 0; ci < b->plan.nclass; ++ci) {
@@ -451,12 +462,15 @@ int build_plan(p3x_batch* b) {
     sc.sets = b->d_sets_sed + lc.set_off;
     sc.first = b->d_first_sed + lc.first_off;
     sc.nsets = lc.nsets;
-    int r12 = 1, rsed = 1;
-    e = occupancy(b->opt.precision, sh, sc.L, &r12, &rsed);
+    int r1 = 1, r2 = 1, rsed = 1;
+    e = occupancy(b->opt.precision, sh, sc.L, &r1, &r2, &rsed);
     if (e == hipErrorInvalidValue) return P3X_ERR_ARG;
     if (e != hipSuccess) return P3X_ERR_DEVICE;
     sc.grid = grid_for(c->ncu, b->opt.blocks_per_cu, lc.total, rsed);
-    if (ci == 0) b->plan.grid = grid_for(c->ncu, b->opt.blocks_per_cu, lay.first[n], r12);
+    if (ci == 0) {
+      b->plan.grid = grid_for(c->ncu, b->opt.blocks_per_cu, lay.first[n], r1);
+      b->plan.grid2 = grid_for(c->ncu, b->opt.blocks_per_cu, lay.first2[n], r2);
+    }
   }
 
   if (b->opt.use_graph) {

@@ -47,7 +47,8 @@ int resolve_shape(int precision, int nk_max, int cb, int nt, int lpc, int wpe, p
 
 // Host-side layout of a batch of column sets for one shape.
 struct Layout {
-  std::vector<int> first;      // part 1 + part 2 virtual-block offsets, nsets + 1
+  std::vector<int> first;      // part 1 virtual-block offsets, nsets + 1
+  std::vector<int> first2;     // part 2 tile offsets, nsets + 1
   std::vector<int> sed_order;  // set indices grouped by sedimentation class (ascending L)
   std::vector<int> sed_first;  // per class nsets_c + 1 offsets, classes concatenated
   struct Cls {
@@ -59,7 +60,7 @@ int layout_batch(const p3d::KParams* sets, int n, const p3d::Shape& sh, Layout* 
 // Persistent grid for `total` virtual blocks: blocks_per_cu x CUs (0: the
 // resident count from the occupancy API; -1: one block per virtual block).
 int grid_for(int ncu, int blocks_per_cu, int total, int resident);
-hipError_t occupancy(int precision, const p3d::Shape& sh, int L, int* r12, int* rsed);
+hipError_t occupancy(int precision, const p3d::Shape& sh, int L, int* r1, int* r2, int* rsed);
 hipError_t launch_plan(int precision, const p3d::BatchPlan& plan, hipStream_t s);
 
 }  // namespace p3x_detail

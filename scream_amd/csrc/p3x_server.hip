@@ -65,7 +65,7 @@ struct p3x_server {
   p3x_runtime rt{};
   p3x_server_options opt{};
   p3d::Shape shape{};
-  int occ12 = 1;
+  int occ1 = 1, occ2 = 1;  // resident blocks per CU of the part 1 and part 2 kernels
   int occ_sed[p3d::kMaxLevelsPerLane + 1] = {};
   struct Member {
     p3d::KParams k;
@@ -92,7 +92,7 @@ size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 // Bytes of descriptors a batch of n sets needs in a slot.
 size_t desc_bytes_for(int n) {
-  return align256(2 * (size_t)n * sizeof(p3d::KParams)) + align256((size_t)(n + 1) * sizeof(int)) +
+  return align256(2 * (size_t)n * sizeof(p3d::KParams)) + 2 * align256((size_t)(n + 1) * sizeof(int)) +
          align256((size_t)(n + p3d::kMaxLevelsPerLane) * sizeof(int));
 }
 
@@ -114,15 +114,17 @@ int launch_slot(p3x_server* srv, Slot& sl) {
   Layout lay;
   int st = layout_batch(sets.data(), n, srv->shape, &lay);
   if (st != P3X_OK) return st;
-  // staging layout: [sets][sed sets][first][sed_first]
+  // staging layout: [sets][sed sets][first][first2][sed_first]
   char* h = sl.h_stage;
   const size_t o_sets = 0, o_sed = (size_t)n * sizeof(p3d::KParams);
   const size_t o_first = align256(2 * (size_t)n * sizeof(p3d::KParams));
-  const size_t o_sfirst = o_first + align256((size_t)(n + 1) * sizeof(int));
+  const size_t o_first2 = o_first + align256((size_t)(n + 1) * sizeof(int));
+  const size_t o_sfirst = o_first2 + align256((size_t)(n + 1) * sizeof(int));
   std::memcpy(h + o_sets, sets.data(), n * sizeof(p3d::KParams));
   for (int i = 0; i < n; ++i)
     std::memcpy(h + o_sed + i * sizeof(p3d::KParams), &sets[lay.sed_order[i]], sizeof(p3d::KParams));
   std::memcpy(h + o_first, lay.first.data(), lay.first.size() * sizeof(int));
+  std::memcpy(h + o_first2, lay.first2.data(), lay.first2.size() * sizeof(int));
   std::memcpy(h + o_sfirst, lay.sed_first.data(), lay.sed_first.size() * sizeof(int));
   const size_t bytes = o_sfirst + lay.sed_first.size() * sizeof(int);
 
@@ -130,8 +132,10 @@ int launch_slot(p3x_server* srv, Slot& sl) {
   plan.shape = srv->shape;
   plan.sets = (const p3d::KParams*)(sl.d_desc + o_sets);
   plan.first = (const int*)(sl.d_desc + o_first);
+  plan.first2 = (const int*)(sl.d_desc + o_first2);
   plan.nsets = n;
-  plan.grid = grid_for(srv->ctx->ncu, 0, lay.first[n], srv->occ12);
+  plan.grid = grid_for(srv->ctx->ncu, 0, lay.first[n], srv->occ1);
+  plan.grid2 = grid_for(srv->ctx->ncu, 0, lay.first2[n], srv->occ2);
   plan.nclass = (int)lay.cls.size();
   for (int c = 0; c < plan.nclass; ++c) {
     const Layout::Cls& lc = lay.cls[c];
@@ -299,9 +303,10 @@ extern "C" int p3x_server_create(p3x_context* c, const p3x_runtime* rt, const p3
   hipError_t e = hipSuccess;
   for (int L = 1; L <= p3d::kMaxLevelsPerLane && e == hipSuccess; ++L) {
     if (L == 7) continue;
-    int r12 = 1, rsed = 1;
-    e = occupancy(o.precision, s->shape, L, &r12, &rsed);
-    s->occ12 = r12;
+    int r1 = 1, r2 = 1, rsed = 1;
+    e = occupancy(o.precision, s->shape, L, &r1, &r2, &rsed);
+    s->occ1 = r1;
+    s->occ2 = r2;
     s->occ_sed[L] = rsed;
   }
   s->occ_sed[7] = s->occ_sed[8];
