@@ -60,21 +60,27 @@ class Workload:
     """Device-resident fields for one column set, plus a pristine copy of the state."""
 
     def __init__(self, ncol, nlev, seed):
+        import numpy as np
         import torch
         from scream_amd import p3x
         from scream_amd.inputs import make_columns
         self.ncol, self.nlev = ncol, nlev
         dev = torch.device("cuda:0")
         inp = make_columns(ncol, nlev, seed=seed)
-        self.f = {k: torch.from_numpy(inp[k]).to(dev) for k in p3x.STATE + p3x.INPUTS}
+        # the 10 state fields are views of one allocation, so that restoring the
+        # pristine state before a step is a single device copy
+        ns = len(p3x.STATE)
+        self.state = torch.from_numpy(np.stack([inp[k] for k in p3x.STATE])).to(dev)
+        self.f = {k: self.state[j] for j, k in enumerate(p3x.STATE)}
+        self.f.update({k: torch.from_numpy(inp[k]).to(dev) for k in p3x.INPUTS})
         for k in p3x.OUTS + p3x.HIST:
             shape = (ncol,) if "surf" in k else ((nlev + 1, ncol) if "flux" in k else (nlev, ncol))
             self.f[k] = torch.zeros(shape, dtype=torch.float64, device=dev)
-        self.pristine = {k: self.f[k].clone() for k in p3x.STATE}
+        self.pristine = self.state.clone()
+        assert self.state.shape[0] == ns
 
     def restore(self):
-        for k, v in self.pristine.items():
-            self.f[k].copy_(v)
+        self.state.copy_(self.pristine)
 
 
 def time_steps(ctx, w, dt, precision, steps, warmup):
@@ -349,7 +355,7 @@ def summarize_pmc(fetch_dir, write_dir):
     scale_f = COPY_PROBE_BYTES / copy_f  # bytes per counter KiB, reads
     scale_w = COPY_PROBE_BYTES / copy_w  # bytes per counter KiB, writes
     kern = {}
-    for name in ("p3_part12_kernel", "p3_sed_kernel"):
+    for name in ("p3_part1_kernel", "p3_part2_kernel", "p3_sed_kernel"):
         f = pick(fetch, name)
         w = pick(write, name)
         if f and w:
