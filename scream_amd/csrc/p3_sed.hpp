@@ -45,20 +45,31 @@ __device__ __forceinline__ double grp_maxd(double v) {
   return v;
 }
 
+template <int LPC>
+__device__ __forceinline__ double grp_sumd(double v) {
+#pragma unroll
+  for (int o = LPC / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
 // Per-lane geometry of the owned levels.
 template <int L>
 struct Geo {
   double rho[L], inv_rho[L], inv_dz[L];
 };
 
-// One species' sedimentation over the group's column.  SP = 0 cloud, 1 rain,
-// 2 ice.  On return the species' fields hold the sedimented values (stored to
-// HBM), pflux accumulates the rain flux at the interface below each owned level,
-// and prt is the mass that left through the surface.
+// One species' sedimentation over the group's column (SP = 0 cloud, 1 rain,
+// 2 ice), with lane j of the group owning the L levels k_top + j*L ..
+// k_top + j*L + L-1 from the species' top level k_top down: sedimentation only
+// moves mass down, so the levels above k_top take no part, and spreading only
+// [k_top, nlev) over the lanes keeps them busy (rain, below the freezing
+// level, fills a quarter of a column).  Loads the species' fields in that
+// mapping, runs the substeps and stores the fields; the rain run also stores
+// the rain flux at the interface below each owned level.  prt (every lane)
+// is the mass that left through the surface.
 template <int SP, class R, int LPC, int L>
-__device__ __forceinline__ void sediment(const KParams& p, int i, int kbase, const Geo<L>& g,
-                                         double (&pflux)[L], double& prt, int& nsub_out,
-                                         bool& present_out, unsigned& err) {
+__device__ __forceinline__ void sediment_run(const KParams& p, int i, int j, int k_top, double& prt,
+                                          int& nsub_out, unsigned& err) {
   constexpr int NF = SP == 2 ? 4 : 2;  // fields held (cloud always holds nc: its DSD rewrites it)
   const int nlev = p.nlev, kbot = nlev - 1;
   const size_t ld = (size_t)p.ld;
@@ -70,36 +81,37 @@ __device__ __forceinline__ void sediment(const KParams& p, int i, int kbase, con
   if (SP == 2) { fld[0] = p.qi; fld[1] = p.ni; fld[2] = p.qm; fld[3] = p.bm; }
   const double* cfp = SP == 0 ? p.cld_frac_l : (SP == 1 ? p.cld_frac_r : p.cld_frac_i);
   const int coef_f = SP == 0 ? SC_ACN : (SP == 1 ? SC_RHOFACR : SC_RHOFACI);
+  const int kbase = k_top + j * L;
 
-  double q[NF][L], cf[L], coef[L], cfl_ice[L];
-  int my_top = nlev, my_bot = -1;
+  Geo<L> g;
+  double q[NF][L], cf[L], coef[L], cfl_ice[L], pflux[L];
+  int my_bot = -1;
 #pragma unroll
   for (int t = 0; t < L; ++t) {
     const int k = kbase + t;
     const bool ok = k < nlev;
     const size_t x = (size_t)(ok ? k : 0) * ld + i;
+    const double dz = ok ? p.dz[x] : 1.0;
+    const double dp = ok ? p.dpres[x] : 1.0;
+    g.rho[t] = dp / dz / gravit;
+    g.inv_rho[t] = 1 / g.rho[t];
+    g.inv_dz[t] = 1 / dz;
 #pragma unroll
     for (int f = 0; f < NF; ++f) q[f][t] = ok ? fld[f][x] : 0.0;
     cf[t] = ok ? cfp[x] : 1.0;
     coef[t] = ok ? p.scratch[((size_t)coef_f * nlev + (ok ? k : 0)) * ld + i] : 0.0;
     cfl_ice[t] = (SP == 2 && ok) ? p.cld_frac_l[x] : 1.0;
-    // find_top / find_bottom, P3I/p3_find_impl.hpp:14-92
-    if (ok && q[0][t] >= QSMALL) {
-      if (k < my_top) my_top = k;
-      my_bot = k;
-    }
+    pflux[t] = 0;
+    // find_bottom, P3I/p3_find_impl.hpp:52-92 (find_top gave k_top)
+    if (ok && q[0][t] >= QSMALL) my_bot = k;
   }
-  const int k_top = grp_min<LPC>(my_top);
   int k_bot = grp_max<LPC>(my_bot);
-  const bool present = k_top < nlev;
-  present_out = present;
-  const int j = kbase / L;
 
   double dt_left = p.dt, prt_accum = 0;
   bool first = true;
   int nsub = 0;
 #pragma unroll 1
-  while (present && dt_left > dt_left_tol) {
+  while (dt_left > dt_left_tol) {
     // The substep count is bounded by the CFL number (a few hundred at most for
     // physical inputs); the cap only guarantees termination for corrupt inputs.
     if (nsub >= kMaxSubsteps) { err |= ERR_SUBSTEP_LIMIT; break; }
@@ -253,19 +265,52 @@ __device__ __forceinline__ void sediment(const KParams& p, int i, int kbase, con
     nsub++;
     first = false;
   }
-  if (present) {
 #pragma unroll
-    for (int t = 0; t < L; ++t) {
-      const int k = kbase + t;
-      if (k < nlev) {
-        const size_t x = (size_t)k * ld + i;
+  for (int t = 0; t < L; ++t) {
+    const int k = kbase + t;
+    if (k < nlev) {
+      const size_t x = (size_t)k * ld + i;
 #pragma unroll
-        for (int f = 0; f < NF; ++f) fld[f][x] = q[f][t];
-      }
+      for (int f = 0; f < NF; ++f) fld[f][x] = q[f][t];
+      if (SP == 1) p.precip_liq_flux[x + ld] = pflux[t];  // rain_sed :140-153, interface k+1
     }
   }
-  prt = prt_accum;
+  prt = grp_sumd<LPC>(prt_accum);  // only the lane owning kbot accumulated
   nsub_out = nsub;
+}
+
+// One species: find_top (P3I/p3_find_impl.hpp:14-50) over the group's
+// column in the standard mapping (lane j: levels j*L .. j*L+L-1), then the
+// substeps on the fewest levels per lane that cover [k_top, nlev).  Every lane
+// of the wavefront calls this (run = its column takes part); k_top_out = nlev
+// when the species is absent.
+template <int SP, class R, int LPC, int L>
+__device__ __forceinline__ void sediment(const KParams& p, int i, int j, bool run, double& prt, int& nsub_out,
+                                         int& k_top_out, unsigned& err) {
+  const int nlev = p.nlev;
+  const size_t ld = (size_t)p.ld;
+  const double* q0 = SP == 0 ? p.qc : (SP == 1 ? p.qr : p.qi);
+  int my_top = nlev;
+  if (run) {
+#pragma unroll
+    for (int t = 0; t < L; ++t) {
+      const int k = j * L + t;
+      if (k < nlev && q0[(size_t)k * ld + i] >= QSMALL && k < my_top) my_top = k;
+    }
+  }
+  const int k_top = grp_min<LPC>(my_top);
+  k_top_out = k_top;
+  prt = 0;
+  nsub_out = 0;
+  // levels per lane for [k_top, nlev), the largest over the wavefront's
+  // columns (all its lanes are here), so that the wavefront takes one path
+  int lw = k_top < nlev ? (nlev - k_top + LPC - 1) / LPC : 0;
+#pragma unroll
+  for (int o = 32; o >= LPC; o >>= 1) lw = max(lw, __shfl_xor(lw, o, 64));
+  if (k_top >= nlev) return;
+  if (lw <= 1) sediment_run<SP, R, LPC, 1>(p, i, j, k_top, prt, nsub_out, err);
+  else if (L > 2 && lw <= 2) sediment_run<SP, R, LPC, (L > 2 ? 2 : L)>(p, i, j, k_top, prt, nsub_out, err);
+  else sediment_run<SP, R, LPC, L>(p, i, j, k_top, prt, nsub_out, err);
 }
 
 // homogeneous_freezing (P3I/p3_ice_sed_impl.hpp:207-270) + p3_main_part3
@@ -406,39 +451,26 @@ __device__ __forceinline__ void sed_block(const KParams& p, int bid) {
   const size_t ld = (size_t)p.ld;
   unsigned err = 0;
 
-  // Whole groups skip together: the cross-lane operations below stay inside a group.
-  if (i < p.ncol) {
+  // Every lane of the wavefront takes part in sediment() (its wavefront-wide
+  // choice of levels per lane); columns past ncol or without a full step only
+  // skip the arithmetic.
+  const bool live = i < p.ncol;
+  const int status = live ? p.colstat[i] : 0;
+  const bool run = live && status == kStatusFull;
+  double prt_c, prt_r, prt_i;
+  int ns_c, ns_r, ns_i, top_c, top_r, top_i;
+  sediment<0, R, LPC, L>(p, i, j, run, prt_c, ns_c, top_c, err);
+  sediment<1, R, LPC, L>(p, i, j, run, prt_r, ns_r, top_r, err);
+  sediment<2, R, LPC, L>(p, i, j, run, prt_i, ns_i, top_i, err);
+  if (live) {
     const double inv_dt = 1 / p.dt;
-    const int status = p.colstat[i];
-    const bool active = status == kStatusFull;
-    double pflux[L];
-#pragma unroll
-    for (int t = 0; t < L; ++t) pflux[t] = 0;
-    double prt_c = 0, prt_r = 0, prt_i = 0;
-    int ns_c = 0, ns_r = 0, ns_i = 0;
-    bool pres_c = false, pres_r = false, pres_i = false;
-    if (active) {
-      Geo<L> g;
-#pragma unroll
-      for (int t = 0; t < L; ++t) {
-        const int k = kbase + t;
-        const size_t x = (size_t)(k < nlev ? k : 0) * ld + i;
-        const double dz = k < nlev ? p.dz[x] : 1.0;
-        const double dp = k < nlev ? p.dpres[x] : 1.0;
-        g.rho[t] = dp / dz / gravit;
-        g.inv_rho[t] = 1 / g.rho[t];
-        g.inv_dz[t] = 1 / dz;
-      }
-      sediment<0, R, LPC, L>(p, i, kbase, g, pflux, prt_c, ns_c, pres_c, err);
-      sediment<1, R, LPC, L>(p, i, kbase, g, pflux, prt_r, ns_r, pres_r, err);
-      sediment<2, R, LPC, L>(p, i, kbase, g, pflux, prt_i, ns_i, pres_i, err);
-    }
-    // rain flux at the interface below each level (rain_sed, P3I/p3_rain_sed_impl.hpp:140-153)
+    const bool pres_c = top_c < nlev, pres_r = top_r < nlev, pres_i = top_i < nlev;
 #pragma unroll
     for (int t = 0; t < L; ++t) {
       const int k = kbase + t;
       if (k >= nlev) continue;
-      p.precip_liq_flux[(size_t)(k + 1) * ld + i] = pflux[t];
+      // no rain flux above the rain's top level (the rain run stored the rest)
+      if (k < top_r) p.precip_liq_flux[(size_t)(k + 1) * ld + i] = 0;
       if (k == kbot) {
         double liq = 0, ice = 0;
         if (pres_c) liq = prt_c * INV_RHO_H2O * inv_dt;
