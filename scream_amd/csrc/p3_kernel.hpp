@@ -29,6 +29,20 @@ namespace p3d {
 // results of part 1 to part 2 and to the sedimentation kernel.
 enum Scr { SC_T = 0, SC_QVSL, SC_QVSI, SC_SUPI, SC_RHOFACR, SC_RHOFACI, SC_ACN, SC_COUNT };
 
+// The P3Microphysics pre- and post-amble (eamxx_p3_process_interface.hpp:63-261),
+// fused into part 1 and part 3 when `on`.  The prognostic pointers then hold the
+// field manager's wet mixing ratios: part 1 reads them wet and writes them dry,
+// part 3 writes them wet again; th is the process's own array.
+struct Amble {
+  int on;
+  const double *p_mid, *dp_wet, *cldfrac_tot;  // p_mid, pseudo_density, cldfrac_tot
+  double *T_mid, *T_prev;                      // T_mid (in, out), T_prev_micro_step (out)
+  double* qv_prev;                             // qv_prev_micro_step: wet in, dry in p3_main, wet out
+  double *dz, *inv_exner, *cld_frac_l, *cld_frac_i, *rainfrac;  // P3 inputs part 1 writes
+  double *precip_liq_mass, *precip_ice_mass;   // per column, accumulated
+  double *vapor_flux, *water_flux, *ice_flux, *heat_flux;  // per column; NULL = off
+};
+
 struct KParams {
   // P3PrognosticState (in/out)
   double *qc, *nc, *qr, *nr, *qi, *qm, *ni, *bm, *qv, *th;
@@ -51,7 +65,12 @@ struct KParams {
   double dt;
   int predictNc, prescribedCCN, exch_zero;
   double e_melt;        // saturation vapour pressure over liquid at Tmelt (qv_sat_dry(Tmelt, p) numerator)
+  Amble amble;
 };
+
+__host__ __device__ __forceinline__ double exner_function(double pres) {  // PhysicsFunctions, p^(Rd/cp)
+  return p3m_pow(pres / P0, Rair * INV_CP);
+}
 
 constexpr int kMaxSubsteps = 100000;
 // Column status bits: part 1 found work, part 2 left hydrometeors.
@@ -96,15 +115,48 @@ __device__ __forceinline__ void part1_block(const KParams& p, int bid, int* s_i)
   if (col_ok) {
     for (int k = k0; k < k1; ++k) {
       const size_t x = at(k);
-      const double inv_exner = p.inv_exner[x];
+      double inv_exner, th, qv, qc, nc, qr, nr, qi, ni, qm, bm, dz, cfl;
+      if (p.amble.on) {
+        // p3_preamble, eamxx_p3_process_interface.hpp:63-124
+        const Amble& a = p.amble;
+        const double pmid = a.p_mid[x], T = a.T_mid[x], cft = a.cldfrac_tot[x];
+        const double dp = a.dp_wet[x], dp_dry = p.dpres[x];
+        const double c1 = -1.0 + 1.0 / ep_2;  // calculate_dz with the wet qv's virtual temperature
+        const double T_virtual = T * (1.0 + c1 * p.qv[x]);
+        dz = (Rair / gravit) * dp * T_virtual / pmid;
+        a.dz[x] = dz;
+        // wet -> dry mixing ratios (calculate_drymmr_from_wetmmr_dp_based)
+        qc = p.qc[x] * dp / dp_dry; nc = p.nc[x] * dp / dp_dry; qr = p.qr[x] * dp / dp_dry;
+        nr = p.nr[x] * dp / dp_dry; qi = p.qi[x] * dp / dp_dry; ni = p.ni[x] * dp / dp_dry;
+        qm = p.qm[x] * dp / dp_dry; bm = p.bm[x] * dp / dp_dry;
+        qv = vmax(p.qv[x] * dp / dp_dry, 0.0);
+        a.qv_prev[x] = a.qv_prev[x] * dp / dp_dry;
+        inv_exner = 1.0 / exner_function(pmid);
+        a.inv_exner[x] = inv_exner;
+        th = T / exner_function(pmid);
+        // cloud fractions with the 1e-4 floor and the maximum-overlap rain fraction
+        cfl = cft > 0.0001 ? cft : 0.0001;
+        a.cld_frac_l[x] = cfl;
+        a.cld_frac_i[x] = cfl;
+        double cfr = cfl;
+        if (k != 0) {
+          const double above = a.cldfrac_tot[x - ld];
+          cfr = above > cfr ? above : cfr;
+        }
+        a.rainfrac[x] = cfr;
+      } else {
+        inv_exner = p.inv_exner[x];
+        th = p.th[x];
+        qv = vmax(p.qv[x], 0.0);
+        qc = p.qc[x]; nc = p.nc[x]; qr = p.qr[x]; nr = p.nr[x];
+        qi = p.qi[x]; ni = p.ni[x]; qm = p.qm[x]; bm = p.bm[x];
+        dz = p.dz[x];
+        cfl = p.prescribedCCN ? p.cld_frac_l[x] : 1.0;
+      }
       const double exner = 1 / inv_exner;
-      double th = p.th[x];
-      double qv = vmax(p.qv[x], 0.0);
-      double qc = p.qc[x], nc = p.nc[x], qr = p.qr[x], nr = p.nr[x];
-      double qi = p.qi[x], ni = p.ni[x], qm = p.qm[x], bm = p.bm[x];
       const double pres = p.pres[x];
       double T_atm = th * exner;
-      const double rho = p.dpres[x] / p.dz[x] / gravit;
+      const double rho = p.dpres[x] / dz / gravit;
       const double inv_rho = 1 / rho;
       const double qv_sat_l = qv_sat_dry(T_atm, pres, false, err);
       const double qv_sat_i = qv_sat_dry(T_atm, pres, true, err);
@@ -125,7 +177,7 @@ __device__ __forceinline__ void part1_block(const KParams& p, int bid, int* s_i)
         qc = 0; nc = 0;
       } else {
         flags |= 2;
-        if (p.prescribedCCN) nc = vmax(nc, rt.spa_ccn_to_nc_factor * p.nccn[x] / (1 / p.cld_frac_l[x]));
+        if (p.prescribedCCN) nc = vmax(nc, rt.spa_ccn_to_nc_factor * p.nccn[x] / (1 / cfl));
         else if (p.predictNc) nc = vmax(nc + p.nc_nuceat_tend[x] * dt, 0.0);
         else nc = NCCNST * inv_rho;
       }

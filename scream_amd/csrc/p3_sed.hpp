@@ -326,9 +326,12 @@ __device__ __forceinline__ void part3_level(const KParams& p, int k, int i, bool
   p.precip_ice_flux[x] = 0;
   if (k == nlev - 1) p.precip_ice_flux[x + ld] = 0;
   double reff_c = 10.e-6, reff_i = 25.e-6, reff_r = 500.e-6, rho_qi = 0;
+  double qc = 0, nc = 0, qr = 0, nr = 0, qi = 0, ni = 0, qm = 0, bm = 0, qv = 0, th = 0;
+  if (active || p.amble.on) {
+    qc = p.qc[x]; nc = p.nc[x]; qr = p.qr[x]; nr = p.nr[x]; qi = p.qi[x]; ni = p.ni[x];
+    qm = p.qm[x]; bm = p.bm[x]; qv = p.qv[x]; th = p.th[x];
+  }
   if (active) {
-    double qc = p.qc[x], nc = p.nc[x], qr = p.qr[x], nr = p.nr[x], qi = p.qi[x], ni = p.ni[x];
-    double qm = p.qm[x], bm = p.bm[x], qv = p.qv[x], th = p.th[x];
     const double inv_exner = p.inv_exner[x];
     if (rt.do_ice_production) {
       const bool t_lt = p.scratch[((size_t)SC_T * nlev + k) * ld + i] < T_homogfrz;
@@ -414,16 +417,32 @@ __device__ __forceinline__ void part3_level(const KParams& p, int k, int i, bool
       }
     }
     if (qr < QSMALL) nr = 0;
-    p.qc[x] = qc; p.nc[x] = nc; p.qr[x] = qr; p.nr[x] = nr; p.qi[x] = qi;
-    p.ni[x] = ni; p.qm[x] = qm; p.bm[x] = bm; p.qv[x] = qv; p.th[x] = th;
     p.vap_liq_exchange[x] = vap_liq;
   } else if (p.dbg_ice_idx) {
     p.dbg_ice_idx[x] = -1;
   }
-  p.diag_eff_radius_qc[x] = reff_c;
-  p.diag_eff_radius_qi[x] = reff_i;
-  p.diag_eff_radius_qr[x] = reff_r;
-  p.rho_qi[x] = rho_qi;
+  if (p.amble.on) {
+    // p3_postamble, eamxx_p3_process_interface.hpp:200-241: the temperature change
+    // rescaled by dp_dry/dp, dry -> wet mixing ratios, radii in microns
+    const Amble& a = p.amble;
+    const double dp = a.dp_wet[x], dp_dry = p.dpres[x];
+    const double T_before = a.T_mid[x];
+    double T = (th * exner_function(a.p_mid[x]) - T_before) * dp_dry / dp;
+    T += T_before;
+    a.T_mid[x] = T;
+    a.T_prev[x] = T;
+    qv = qv * dp_dry / dp;
+    p.qc[x] = qc * dp_dry / dp; p.nc[x] = nc * dp_dry / dp; p.qr[x] = qr * dp_dry / dp;
+    p.nr[x] = nr * dp_dry / dp; p.qi[x] = qi * dp_dry / dp; p.ni[x] = ni * dp_dry / dp;
+    p.qm[x] = qm * dp_dry / dp; p.bm[x] = bm * dp_dry / dp; p.qv[x] = qv;
+    a.qv_prev[x] = qv;
+    reff_c *= 1e6;
+    reff_i *= 1e6;
+    reff_r *= 1e6;
+  } else if (active) {
+    p.qc[x] = qc; p.nc[x] = nc; p.qr[x] = qr; p.nr[x] = nr; p.qi[x] = qi;
+    p.ni[x] = ni; p.qm[x] = qm; p.bm[x] = bm; p.qv[x] = qv; p.th[x] = th;
+  }
   p.diag_eff_radius_qc[x] = reff_c;
   p.diag_eff_radius_qi[x] = reff_i;
   p.diag_eff_radius_qr[x] = reff_r;
@@ -478,6 +497,17 @@ __device__ __forceinline__ void sed_block(const KParams& p, int bid) {
         if (pres_i) ice += prt_i * INV_RHO_H2O * inv_dt;
         p.precip_liq_surf[i] = liq;
         p.precip_ice_surf[i] = ice;
+        if (p.amble.on) {  // the per-column post-amble, eamxx_p3_process_interface.hpp:245-258
+          const Amble& a = p.amble;
+          a.precip_liq_mass[i] += liq * RHO_H2O * p.dt;
+          a.precip_ice_mass[i] += ice * RHO_H2O * p.dt;
+          if (a.water_flux) {
+            a.vapor_flux[i] = 0.0;
+            a.water_flux[i] = liq + ice;
+            a.ice_flux[i] = ice;
+            a.heat_flux[i] = 0.0;
+          }
+        }
       }
     }
     if (j == 0) {

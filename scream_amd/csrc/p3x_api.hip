@@ -125,6 +125,7 @@ void fill_params(const p3x_context* c, const p3x_runtime* runtime, const p3x_pro
   // portable arithmetic on the host as on the device
   unsigned no_err = 0;
   k.e_melt = p3d::mk_svp(p3c::Tmelt, false, no_err);
+  k.amble = p3d::Amble{};  // no fused pre/post-amble
 }
 
 // Every field pointer of a column set is non-null.
@@ -256,6 +257,13 @@ extern "C" int p3x_main(p3x_context* c, const p3x_runtime* runtime, const p3x_pr
                         const p3x_diagnostic_inputs* di, const p3x_diagnostic_outputs* dout,
                         const p3x_infrastructure* infra, const p3x_history_only* hist, int nj, int nk,
                         const p3x_launch* opts, double* elapsed_us) {
+  return main_impl(c, runtime, ps, di, dout, infra, hist, nj, nk, opts, elapsed_us, nullptr);
+}
+
+int p3x_detail::main_impl(p3x_context* c, const p3x_runtime* runtime, const p3x_prognostic_state* ps,
+                          const p3x_diagnostic_inputs* di, const p3x_diagnostic_outputs* dout,
+                          const p3x_infrastructure* infra, const p3x_history_only* hist, int nj, int nk,
+                          const p3x_launch* opts, double* elapsed_us, const p3d::Amble* amble) {
   if (!c || !runtime || !ps || !di || !dout || !infra || !hist) return P3X_ERR_ARG;
   if (nj < 0 || nk <= 0 || !(infra->dt > 0)) return P3X_ERR_ARG;
   p3x_launch o;
@@ -289,6 +297,7 @@ extern "C" int p3x_main(p3x_context* c, const p3x_runtime* runtime, const p3x_pr
   p3d::KParams k;
   fill_params(c, runtime, ps, di, dout, infra, hist, nj, nk, ld, o.col_status, o.sed_substeps,
               o.ice_table_idx, c->d_scratch, &k);
+  if (amble) k.amble = *amble;
 
   hipError_t e = hipEventRecord(c->ev0, s);
   if (e != hipSuccess) return P3X_ERR_DEVICE;

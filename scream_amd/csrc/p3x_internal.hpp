@@ -30,6 +30,11 @@ int dev_fail(hipError_t e);
 // Default launch shape per (precision, nlev).
 void default_shape(int precision, int nk, p3d::Shape* sh);
 // KParams of one column set; scratch points at SC_COUNT*nk*ld doubles followed by ld ints.
+// p3x_main with (amble != NULL) the P3Microphysics pre/post-amble fused into its kernels.
+int main_impl(p3x_context* c, const p3x_runtime* runtime, const p3x_prognostic_state* ps,
+              const p3x_diagnostic_inputs* di, const p3x_diagnostic_outputs* dout,
+              const p3x_infrastructure* infra, const p3x_history_only* hist, int nj, int nk,
+              const p3x_launch* opts, double* elapsed_us, const p3d::Amble* amble);
 void fill_params(const p3x_context* c, const p3x_runtime* runtime, const p3x_prognostic_state* ps,
                  const p3x_diagnostic_inputs* di, const p3x_diagnostic_outputs* dout,
                  const p3x_infrastructure* infra, const p3x_history_only* hist, int nj, int nk, int ld,

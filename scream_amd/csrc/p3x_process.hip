@@ -1,8 +1,10 @@
 // p3x_process.hip -- the P3Microphysics atmosphere process around p3_main:
 // pre-amble (wet -> dry mixing ratios, dz, exner, theta, cloud fractions with
-// maximum-overlap rain fraction), the two p3_main kernels, and post-amble (the
-// rescaled temperature change, dry -> wet, effective radii in microns, the
-// accumulated surface precipitation masses and the optional boundary fluxes).
+// maximum-overlap rain fraction), p3_main, and post-amble (the rescaled
+// temperature change, dry -> wet, effective radii in microns, the accumulated
+// surface precipitation masses and the optional boundary fluxes).  The pre-amble
+// runs inside the part 1 kernel and the post-amble inside the sedimentation +
+// part 3 kernel (p3d::Amble), so a process step is the three p3_main kernels.
 //
 // Follows components/eamxx/src/physics/p3/eamxx_p3_process_interface.hpp:63-261,
 // eamxx_p3_process_interface.cpp:249-340 (which fields feed p3_main) and
@@ -20,90 +22,6 @@
 using namespace p3x_detail;
 
 namespace {
-
-constexpr double P0 = p3c::P0;
-
-__device__ __forceinline__ double exner_function(double p) {  // :88-98
-  return p3m_pow(p / P0, p3c::Rair * p3c::INV_CP);
-}
-
-struct AmbleParams {
-  p3x_atm_fields f;
-  double *th_atm, *inv_exner, *dz, *cld_frac_l, *cld_frac_i;
-  const double *precip_liq_surf, *precip_ice_surf;
-  int ncol, nlev;
-  double dt;
-};
-
-// One thread per (level, column).
-__global__ __launch_bounds__(256) void p3_preamble_kernel(const AmbleParams a) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int k = blockIdx.y;
-  if (i >= a.ncol) return;
-  const size_t x = (size_t)k * a.ncol + i;
-  const p3x_atm_fields& f = a.f;
-  const double pmid = f.p_mid[x], T = f.T_mid[x], cft = f.cldfrac_tot[x];
-  const double dp = f.pseudo_density[x], dp_dry = f.pseudo_density_dry[x];
-  // calculate_dz with the virtual temperature of the wet qv (:77; :214-224, :400-411)
-  const double c1 = -1.0 + 1.0 / p3c::ep_2;
-  const double T_virtual = T * (1.0 + c1 * f.qv[x]);
-  a.dz[x] = (p3c::Rair / p3c::gravit) * dp * T_virtual / pmid;
-  // wet -> dry, calculate_drymmr_from_wetmmr_dp_based (:86-98; :375-381)
-  double* const q[] = {f.qc, f.nc, f.qr, f.nr, f.qi, f.ni, f.qm, f.bm, f.qv, f.qv_prev_micro_step};
-#pragma unroll
-  for (int j = 0; j < 10; ++j) q[j][x] = q[j][x] * dp / dp_dry;
-  // exner and theta from the full pressure (:101-104; :115-118)
-  const double exner = exner_function(pmid);
-  a.inv_exner[x] = 1.0 / exner;
-  a.th_atm[x] = T / exner_function(pmid);
-  // cloud fractions with the 1e-4 floor and maximum-overlap rain fraction (:107-124)
-  const double mincld = 0.0001;
-  const double cf = cft > mincld ? cft : mincld;
-  a.cld_frac_l[x] = cf;
-  a.cld_frac_i[x] = cf;
-  double cfr = cf;
-  if (k != 0) {
-    const double above = f.cldfrac_tot[x - a.ncol];
-    cfr = above > cfr ? above : cfr;
-  }
-  f.rainfrac[x] = cfr;
-}
-
-__global__ __launch_bounds__(256) void p3_postamble_kernel(const AmbleParams a) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int k = blockIdx.y;
-  if (i >= a.ncol) return;
-  const p3x_atm_fields& f = a.f;
-  if (k < a.nlev) {
-    const size_t x = (size_t)k * a.ncol + i;
-    const double dp = f.pseudo_density[x], dp_dry = f.pseudo_density_dry[x];
-    // rescaled temperature change (:210-217; calculate_T_from_theta :162-165)
-    const double T_before = f.T_mid[x];
-    double T = (a.th_atm[x] * exner_function(f.p_mid[x]) - T_before) * dp_dry / dp;
-    T += T_before;
-    f.T_mid[x] = T;
-    f.T_prev_micro_step[x] = T;
-    // dry -> wet, calculate_wetmmr_from_drymmr_dp_based (:226-234; :327-333)
-    double* const q[] = {f.qc, f.nc, f.qr, f.nr, f.qi, f.ni, f.qm, f.bm, f.qv};
-#pragma unroll
-    for (int j = 0; j < 9; ++j) q[j][x] = q[j][x] * dp_dry / dp;
-    f.qv_prev_micro_step[x] = f.qv[x];
-    // effective radii in microns (:238-240)
-    f.eff_radius_qc[x] *= 1e6;
-    f.eff_radius_qi[x] *= 1e6;
-    f.eff_radius_qr[x] *= 1e6;
-  } else {
-    // the per-column part (:245-258), on an extra row of the grid
-    f.precip_liq_surf_mass[i] += a.precip_liq_surf[i] * p3c::RHO_H2O * a.dt;
-    f.precip_ice_surf_mass[i] += a.precip_ice_surf[i] * p3c::RHO_H2O * a.dt;
-    if (f.water_flux) {
-      f.vapor_flux[i] = 0.0;
-      f.water_flux[i] = a.precip_liq_surf[i] + a.precip_ice_surf[i];
-      f.ice_flux[i] = a.precip_ice_surf[i];
-      f.heat_flux[i] = 0.0;
-    }
-  }
-}
 
 struct BudgetParams {
   p3x_budget_fields f;
@@ -239,24 +157,8 @@ extern "C" int p3x_process_run(p3x_process* p, const p3x_atm_fields* f, double d
   if (o.ld != 0 && o.ld != p->ncol) return P3X_ERR_ARG;  // the process's arrays are dense
   hipStream_t s = (hipStream_t)o.stream;
 
-  AmbleParams a;
-  a.f = *f;
-  a.th_atm = p->th_atm;
-  a.inv_exner = p->inv_exner;
-  a.dz = p->dz;
-  a.cld_frac_l = p->cld_frac_l;
-  a.cld_frac_i = p->cld_frac_i;
-  a.precip_liq_surf = p->precip_liq_surf;
-  a.precip_ice_surf = p->precip_ice_surf;
-  a.ncol = p->ncol;
-  a.nlev = p->nlev;
-  a.dt = dt;
-  const dim3 blk(256), grid((p->ncol + 255) / 256, p->nlev);
-  hipLaunchKernelGGL(p3_preamble_kernel, grid, blk, 0, s, a);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return P3X_ERR_DEVICE;
-
-  // p3_main on the process's views (eamxx_p3_process_interface.cpp:284-337)
+  // p3_main on the process's views (eamxx_p3_process_interface.cpp:284-337); the
+  // pre-amble runs inside part 1 and the post-amble inside part 3.
   p3x_prognostic_state ps = {f->qc, f->nc, f->qr, f->nr, f->qi, f->qm, f->ni, f->bm, f->qv, p->th_atm};
   p3x_diagnostic_inputs di = {f->nc_nuceat_tend, p->prescribedCCN ? f->nccn : p->unused, f->ni_activated,
                               f->inv_qc_relvar, p->cld_frac_i, p->cld_frac_l, f->rainfrac, f->p_dry_mid,
@@ -269,15 +171,29 @@ extern "C" int p3x_process_run(p3x_process* p, const p3x_atm_fields* f, double d
   // run_impl zeroes the three exchange fields before p3_main (eamxx_p3_run.cpp:27-29)
   p3x_infrastructure inf = {dt, ++p->it, 0, p->ncol - 1, 0, p->nlev - 1, p->predictNc, p->prescribedCCN,
                             nullptr, 1};
+  p3d::Amble a{};
+  a.on = 1;
+  a.p_mid = f->p_mid;
+  a.dp_wet = f->pseudo_density;
+  a.cldfrac_tot = f->cldfrac_tot;
+  a.T_mid = f->T_mid;
+  a.T_prev = f->T_prev_micro_step;
+  a.qv_prev = f->qv_prev_micro_step;
+  a.dz = p->dz;
+  a.inv_exner = p->inv_exner;
+  a.cld_frac_l = p->cld_frac_l;
+  a.cld_frac_i = p->cld_frac_i;
+  a.rainfrac = f->rainfrac;
+  a.precip_liq_mass = f->precip_liq_surf_mass;
+  a.precip_ice_mass = f->precip_ice_surf_mass;
+  a.vapor_flux = f->vapor_flux;
+  a.water_flux = f->water_flux;
+  a.ice_flux = f->ice_flux;
+  a.heat_flux = f->heat_flux;
   p3x_launch mo = o;
   mo.async = 1;
-  int st = p3x_main(p->ctx, &p->rt, &ps, &di, &dout, &inf, &hist, p->ncol, p->nlev, &mo, nullptr);
+  const int st = main_impl(p->ctx, &p->rt, &ps, &di, &dout, &inf, &hist, p->ncol, p->nlev, &mo, nullptr, &a);
   if (st != P3X_OK) return st;
-
-  const dim3 grid2((p->ncol + 255) / 256, p->nlev + 1);
-  hipLaunchKernelGGL(p3_postamble_kernel, grid2, blk, 0, s, a);
-  e = hipGetLastError();
-  if (e != hipSuccess) return P3X_ERR_DEVICE;
   if (o.async) return P3X_OK;
   return p3x_sync(p->ctx);
 }
