@@ -41,6 +41,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-forms", action="store_true", help="skip the 16384x128 forms")
+    ap.add_argument("--no-process", action="store_true",
+                    help="skip the P3Microphysics process (run_impl) leg")
     ap.add_argument("--batch-sets", type=int, default=64,
                     help="column sets of the batched-form leg; 0 skips it")
     ap.add_argument("--serve-seconds", type=float, default=5.0,
@@ -126,6 +128,58 @@ def kernel_split(ctx, w, dt, precision, steps):
         a.append(x)
         b.append(y)
     return statistics.mean(a), statistics.mean(b)
+
+
+def process_step(ctx, ncol, nlev, dt, steps, warmup, seed=1):
+    """The P3Microphysics process step (run_impl: pre-amble, p3_main, post-amble, the
+    pre/post-amble fused into the p3_main kernels) on wet field-manager fields of the
+    same generator: wall ms/step and HIP-event kernel us/step, the updated fields
+    restored from one pristine copy before every step (outside the events)."""
+    import numpy as np
+    import torch
+    from scream_amd import p3x
+    from scream_amd.inputs import (ATM_COMPUTED_FIELDS, ATM_FLUX_FIELDS, ATM_UPDATED_FIELDS,
+                                   make_atm_fields)
+    dev = torch.device("cuda:0")
+    f = make_atm_fields(ncol, nlev, seed=seed)
+    flat = torch.from_numpy(np.concatenate([f[k].ravel() for k in ATM_UPDATED_FIELDS])).to(dev)
+    d, off = {}, 0
+    for k in ATM_UPDATED_FIELDS:
+        n = f[k].size
+        d[k] = flat[off:off + n].view(f[k].shape)
+        off += n
+    d.update({k: torch.from_numpy(v).to(dev) for k, v in f.items() if k not in d})
+    for k in ATM_COMPUTED_FIELDS:
+        d[k] = torch.zeros((nlev, ncol), dtype=torch.float64, device=dev)
+    for k in ATM_FLUX_FIELDS:
+        d[k] = torch.zeros((ncol,), dtype=torch.float64, device=dev)
+    pristine = flat.clone()
+    proc = p3x.P3Microphysics(ctx, ncol, nlev, predict_nc=True, prescribed_ccn=False)
+    stream = torch.cuda.current_stream()
+    sp = stream.cuda_stream
+    for _ in range(warmup):
+        flat.copy_(pristine)
+        proc.run(d, dt, stream=sp, async_=True)
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for e0, e1 in evs:
+        flat.copy_(pristine)
+        e0.record(stream)
+        proc.run(d, dt, stream=sp, async_=True)
+        e1.record(stream)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    st = ctx.sync()
+    proc.close()
+    if st != 0:
+        raise RuntimeError(f"p3x status {st}")
+    ms = 1000.0 * (t1 - t0) / steps
+    kern_us = statistics.mean(1000.0 * e0.elapsed_time(e1) for e0, e1 in evs)
+    return {"workload": f"P3Microphysics run_impl FP64, {ncol}x{nlev}, dt={dt:g} s, boundary fluxes on",
+            "column_levels_per_s": ncol * nlev / (ms / 1000.0), "ms_per_step": ms,
+            "kernel_us_per_step": kern_us}
 
 
 def mixed_error(ctx, w, dt):
@@ -434,6 +488,8 @@ def main():
                      "kernel": "p3_sed_kernel"},
     }
     del w
+    if not args.no_process:
+        line["process"] = process_step(ctx, ncol, nlev, dt, args.steps, args.warmup)
     if not args.no_forms:
         ncol2, nlev2, dt2 = 16384, 128, 100.0
         w2 = Workload(ncol2, nlev2, seed=2)

@@ -268,7 +268,9 @@ typedef struct p3x_process p3x_process;
  * precipitation fluxes, qv2qi_depos_tend, rho_qi). */
 int p3x_process_create(p3x_context* ctx, const p3x_runtime* runtime, int ncol, int nlev,
                        int predictNc, int prescribedCCN, p3x_process** process);
-/* run_impl(dt): pre-amble, p3_main, post-amble (same stream, same call). */
+/* run_impl(dt): pre-amble, p3_main, post-amble (same stream, same call); the
+ * pre-amble runs inside p3_main's first kernel and the post-amble inside its
+ * last, so a step is three kernel launches. */
 int p3x_process_run(p3x_process* process, const p3x_atm_fields* fields, double dt,
                     const p3x_launch* opts);
 void p3x_process_destroy(p3x_process* process);
@@ -293,6 +295,18 @@ int p3x_conservation_errors(p3x_context* ctx, const double* mass_before, const d
                             const double* vapor_flux, const double* water_flux, const double* ice_flux,
                             const double* heat_flux, int ncol, double dt, double* mass_err,
                             double* energy_err, void* stream);
+
+/* ---- the CldFraction process that runs before P3 and produces its cldfrac_tot
+ * (physics/cld_fraction/eamxx_cld_fraction_process_interface.cpp:77-90,
+ * cld_fraction_main_impl.hpp:12-95): ice cloud fraction 1 where qi exceeds a
+ * threshold (strictly) and 0 elsewhere, total = max(ice, liquid), for the
+ * model's threshold (default 1e-12) and the analysis threshold (default 1e-5).
+ * Level-major device arrays with ld = ncol; enqueued on stream (NULL =
+ * default), returns at once. ---- */
+int p3x_cld_fraction(p3x_context* ctx, const double* qi, const double* cldfrac_liq, int ncol, int nlev,
+                     double ice_threshold, double ice_4out_threshold, double* cldfrac_ice,
+                     double* cldfrac_tot, double* cldfrac_ice_4out, double* cldfrac_tot_4out,
+                     void* stream);
 
 /* ---- served form (no reference counterpart): step requests from concurrent
  * ensemble members, continuously batched.  Members register once; their

@@ -145,6 +145,7 @@ def load_oracle(math="glibc"):
         lib.p3o_apply_table_coll.argtypes = [C.POINTER(_Tables), C.c_int] + [C.c_double] * 6
         lib.p3o_column_budget.argtypes = [_dp] * 10 + [C.c_int, C.c_int, _dp, _dp]
         lib.p3o_conservation_errors.argtypes = [_dp] * 8 + [C.c_int, C.c_double, _dp, _dp]
+        lib.p3o_cld_fraction.argtypes = [_dp, _dp, C.c_int, C.c_int, C.c_double, C.c_double] + [_dp] * 4
         lib.p3o_preamble.argtypes = [C.POINTER(_Atm), C.c_int, C.c_int]
         lib.p3o_postamble.argtypes = [C.POINTER(_Atm), C.c_int, C.c_int, C.c_double]
         lib.p3o_upwind_step.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_int,
@@ -278,6 +279,23 @@ def column_budget(f: dict, math="glibc"):
     lib.p3o_column_budget(*[(_ptr(a[k]) if a[k] is not None else None) for k in a], ncol, nlev,
                           _ptr(mass), _ptr(energy))
     return mass, energy
+
+
+CLD_FRACTION_OUT = ("cldfrac_ice", "cldfrac_tot", "cldfrac_ice_for_analysis", "cldfrac_tot_for_analysis")
+
+
+def cld_fraction(qi, cldfrac_liq, ice_threshold=1e-12, ice_4out_threshold=1e-5, math="glibc"):
+    """The CldFraction process (p3o_cld_fraction) on level-major (nlev, ncol) arrays:
+    a dict of its four outputs, CLD_FRACTION_OUT."""
+    lib = load_oracle(math)
+    qi = np.ascontiguousarray(qi, dtype=np.float64)
+    liq = np.ascontiguousarray(cldfrac_liq, dtype=np.float64)
+    assert qi.shape == liq.shape and qi.ndim == 2
+    nlev, ncol = qi.shape
+    out = {k: np.zeros_like(qi) for k in CLD_FRACTION_OUT}
+    lib.p3o_cld_fraction(_ptr(qi), _ptr(liq), ncol, nlev, float(ice_threshold), float(ice_4out_threshold),
+                         *[_ptr(out[k]) for k in CLD_FRACTION_OUT])
+    return out
 
 
 def conservation_errors(m0, e0, m1, e1, vapor, water, ice, heat, dt, math="glibc"):

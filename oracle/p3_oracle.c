@@ -1623,6 +1623,27 @@ void p3o_postamble(const p3o_atm* a, int ncol, int nlev, double dt) {
 }
 
 /* ======================================================================
+ * CldFraction (components/eamxx/src/physics/cld_fraction/cld_fraction_main_impl.hpp:
+ *  main :12-50 runs calc_icefrac :52-73 with the two thresholds, then
+ *  calc_totalfrac :75-93; the thresholds come from
+ *  eamxx_cld_fraction_process_interface.cpp:60-62).  The pack max is taken as
+ *  a < b ? b : a.
+ * ====================================================================== */
+void p3o_cld_fraction(const double* qi, const double* cldfrac_liq, int ncol, int nlev, double ice_threshold,
+                      double ice_4out_threshold, double* cldfrac_ice, double* cldfrac_tot,
+                      double* cldfrac_ice_4out, double* cldfrac_tot_4out) {
+  const size_t n = (size_t)ncol * nlev;
+  for (size_t x = 0; x < n; ++x) {
+    const double ice = qi[x] > ice_threshold ? 1.0 : 0.0;
+    const double ice4 = qi[x] > ice_4out_threshold ? 1.0 : 0.0;
+    cldfrac_ice[x] = ice;
+    cldfrac_ice_4out[x] = ice4;
+    cldfrac_tot[x] = ice < cldfrac_liq[x] ? cldfrac_liq[x] : ice;
+    cldfrac_tot_4out[x] = ice4 < cldfrac_liq[x] ? cldfrac_liq[x] : ice4;
+  }
+}
+
+/* ======================================================================
  * Column mass and energy of the reference's conservation property check
  * (components/eamxx/src/share/property_checks/mass_and_energy_column_conservation_check.cpp:
  *  compute_total_mass_on_column :283-301, compute_total_energy_on_column :312-340,

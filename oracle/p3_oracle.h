@@ -124,6 +124,12 @@ void p3o_conservation_errors(const double* mass0, const double* energy0, const d
                              const double* ice_flux, const double* heat_flux, int ncol, double dt,
                              double* mass_err, double* energy_err);
 
+/* The CldFraction process before P3: ice cloud fraction 1 where qi > threshold,
+ * total = max(ice, liquid), for both thresholds.  Level-major, ld = ncol. */
+void p3o_cld_fraction(const double* qi, const double* cldfrac_liq, int ncol, int nlev, double ice_threshold,
+                      double ice_4out_threshold, double* cldfrac_ice, double* cldfrac_tot,
+                      double* cldfrac_ice_4out, double* cldfrac_tot_4out);
+
 /* Defaults of P3Runtime. */
 void p3o_runtime_defaults(p3o_runtime* rt);
 

@@ -23,6 +23,27 @@ using namespace p3x_detail;
 
 namespace {
 
+struct CldFracParams {
+  const double *qi, *liq;
+  double *ice, *tot, *ice4, *tot4;
+  double thr, thr4;
+  size_t n;
+};
+
+// calc_icefrac (strict threshold) and calc_totalfrac for both thresholds,
+// cld_fraction_main_impl.hpp:52-95; one thread per column-level.
+__global__ __launch_bounds__(256) void p3_cld_fraction_kernel(const CldFracParams a) {
+  const size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= a.n) return;
+  const double qi = a.qi[x], liq = a.liq[x];
+  const double ice = qi > a.thr ? 1.0 : 0.0, ice4 = qi > a.thr4 ? 1.0 : 0.0;
+  a.ice[x] = ice;
+  a.ice4[x] = ice4;
+  // max(ice, liq) as a < b ? b : a (std::max's order: a NaN liquid fraction gives ice)
+  a.tot[x] = ice < liq ? liq : ice;
+  a.tot4[x] = ice4 < liq ? liq : ice4;
+}
+
 struct BudgetParams {
   p3x_budget_fields f;
   double *mass, *energy;
@@ -77,6 +98,20 @@ extern "C" int p3x_column_budget(p3x_context* c, const p3x_budget_fields* f, int
   if (ncol == 0) return P3X_OK;
   const BudgetParams b{*f, mass, energy, ncol, nlev};
   hipLaunchKernelGGL(p3_column_budget_kernel, dim3((ncol + 255) / 256), dim3(256), 0, (hipStream_t)stream, b);
+  return hipGetLastError() == hipSuccess ? P3X_OK : P3X_ERR_DEVICE;
+}
+
+extern "C" int p3x_cld_fraction(p3x_context* c, const double* qi, const double* cldfrac_liq, int ncol,
+                                int nlev, double ice_threshold, double ice_4out_threshold,
+                                double* cldfrac_ice, double* cldfrac_tot, double* cldfrac_ice_4out,
+                                double* cldfrac_tot_4out, void* stream) {
+  if (!c || !qi || !cldfrac_liq || !cldfrac_ice || !cldfrac_tot || !cldfrac_ice_4out || !cldfrac_tot_4out ||
+      ncol <= 0 || nlev <= 0)
+    return P3X_ERR_ARG;
+  CldFracParams a{qi, cldfrac_liq, cldfrac_ice, cldfrac_tot, cldfrac_ice_4out, cldfrac_tot_4out,
+                  ice_threshold, ice_4out_threshold, (size_t)ncol * nlev};
+  const size_t blocks = (a.n + 255) / 256;
+  hipLaunchKernelGGL(p3_cld_fraction_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, a);
   return hipGetLastError() == hipSuccess ? P3X_OK : P3X_ERR_DEVICE;
 }
 

@@ -43,7 +43,8 @@ EXPORTS = ("p3x_runtime_defaults", "p3x_launch_defaults", "p3x_tables_load", "p3
            "p3x_server_defaults", "p3x_server_create", "p3x_server_add_member", "p3x_server_submit",
            "p3x_server_poll", "p3x_server_drain", "p3x_server_stats_get", "p3x_server_destroy",
            "p3x_process_create", "p3x_process_run", "p3x_process_destroy", "p3x_column_budget",
-           "p3x_conservation_errors", "p3x_ice_table_generate", "p3x_ice_table_write")
+           "p3x_conservation_errors", "p3x_cld_fraction", "p3x_ice_table_generate",
+           "p3x_ice_table_write")
 
 # p3x_atm_fields, in struct order (the P3Microphysics field names)
 ATM_FIELDS = ("p_mid", "p_dry_mid", "pseudo_density", "pseudo_density_dry", "cldfrac_tot",
@@ -228,6 +229,9 @@ def load() -> C.CDLL:
                                                                                  C.c_void_p, C.c_void_p,
                                                                                  C.c_void_p]
         lib.p3x_conservation_errors.restype = C.c_int
+        lib.p3x_cld_fraction.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_double,
+                                         C.c_double] + [C.c_void_p] * 5
+        lib.p3x_cld_fraction.restype = C.c_int
         _lib = lib
     return _lib
 
@@ -534,6 +538,75 @@ class P3Microphysics:
             self.close()
         except Exception:
             pass
+
+
+CLD_FRACTION_OUT = ("cldfrac_ice", "cldfrac_tot", "cldfrac_ice_for_analysis", "cldfrac_tot_for_analysis")
+
+
+class CldFraction:
+    """The CldFraction process (physics/cld_fraction/eamxx_cld_fraction_process_interface.cpp):
+    parameters ice_cloud_threshold (default 1e-12) and ice_cloud_for_analysis_threshold
+    (default 1e-5), :60-62; ``run(fields)`` reads qi and cldfrac_liq and writes the four
+    CLD_FRACTION_OUT fields (device arrays by field name)."""
+
+    name = "CldFraction"
+
+    def __init__(self, ctx: Context, ncol: int, nlev: int, ice_cloud_threshold: float = 1e-12,
+                 ice_cloud_for_analysis_threshold: float = 1e-5):
+        self._lib = load()
+        self.ctx, self.ncol, self.nlev = ctx, ncol, nlev
+        self.ice_cloud_threshold = float(ice_cloud_threshold)
+        self.ice_cloud_for_analysis_threshold = float(ice_cloud_for_analysis_threshold)
+
+    def run(self, fields: dict, stream=None):
+        st = self._lib.p3x_cld_fraction(self.ctx.handle, _addr(fields["qi"]), _addr(fields["cldfrac_liq"]),
+                                        int(self.ncol), int(self.nlev), self.ice_cloud_threshold,
+                                        self.ice_cloud_for_analysis_threshold,
+                                        *[_addr(fields[k]) for k in CLD_FRACTION_OUT], stream)
+        if st != P3X_OK:
+            raise P3Error(st, "p3x_cld_fraction")
+
+
+class MicrophysicsStep:
+    """The stretch of the physics group around P3 that needs no other neighbour:
+    CldFraction (its cldfrac_tot feeds P3), then P3Microphysics with the mass-and-
+    energy column conservation check around it (the property check the atmosphere
+    driver runs for a process with the check on: budgets before and after run_impl
+    and the relative errors against P3's boundary fluxes).  All on one stream.
+
+    ``run(fields, dt)`` takes the P3 fields of ATM_FIELDS (the boundary fluxes
+    required), qi and cldfrac_liq for CldFraction, the four CLD_FRACTION_OUT
+    outputs (cldfrac_tot is P3's input), and optionally u, v, ps, phis for the
+    energy budget; it returns the per-column (mass_err, energy_err) device arrays.
+    With async_ the call returns once the work is enqueued and p3x_sync reports
+    P3's status; otherwise it waits and raises on an error status."""
+
+    def __init__(self, ctx: Context, ncol: int, nlev: int, predict_nc=True, prescribed_ccn=False,
+                 runtime: Runtime | None = None, ice_cloud_threshold: float = 1e-12,
+                 ice_cloud_for_analysis_threshold: float = 1e-5):
+        import torch
+        self.ctx, self.ncol, self.nlev = ctx, ncol, nlev
+        self.cld = CldFraction(ctx, ncol, nlev, ice_cloud_threshold, ice_cloud_for_analysis_threshold)
+        self.p3 = P3Microphysics(ctx, ncol, nlev, predict_nc, prescribed_ccn, runtime)
+        dev = torch.device("cuda", torch.cuda.current_device())
+        self._b = torch.zeros((6, ncol), dtype=torch.float64, device=dev)  # m0 e0 m1 e1 merr eerr
+
+    def run(self, fields: dict, dt: float, precision: int = P3X_FP64, stream=None, async_: bool = False):
+        m0, e0, m1, e1, me, ee = self._b
+        self.cld.run(fields, stream=stream)
+        column_budget(self.ctx, fields, self.ncol, self.nlev, m0, e0, stream=stream)
+        self.p3.run(fields, dt, precision=precision, stream=stream, async_=True)
+        column_budget(self.ctx, fields, self.ncol, self.nlev, m1, e1, stream=stream)
+        conservation_errors(self.ctx, m0, e0, m1, e1, fields["vapor_flux"], fields["water_flux"],
+                            fields["ice_flux"], fields["heat_flux"], self.ncol, dt, me, ee, stream=stream)
+        if not async_:
+            st = self.ctx.sync()
+            if st != P3X_OK:
+                raise P3Error(st, "MicrophysicsStep.run")
+        return me, ee
+
+    def close(self):
+        self.p3.close()
 
 
 def column_budget(ctx: Context, fields: dict, ncol: int, nlev: int, mass, energy, stream=None):
