@@ -417,7 +417,7 @@ def summarize_pmc(fetch_dir, write_dir):
                           "bytes": max(f.values()) * scale_f + max(w.values()) * scale_w}
     out = {"g64_fp64": {
         "workload": "p3_main FP64 drop-in, 21600x72, dt=1800 s (bench.py --no-cpu --no-forms "
-                    "--batch-sets 0 --serve-seconds 0)",
+                    "--no-process --batch-sets 0 --serve-seconds 0)",
         "copy_probe": {"bytes_each_way": COPY_PROBE_BYTES, "FETCH_SIZE_KiB": copy_f,
                        "WRITE_SIZE_KiB": copy_w, "read_bytes_per_KiB": scale_f,
                        "write_bytes_per_KiB": scale_w},
