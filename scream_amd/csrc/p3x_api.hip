@@ -28,11 +28,15 @@ int dev_fail(hipError_t e) { return e == hipErrorOutOfMemory ? P3X_ERR_NO_MEMORY
 // Sedimentation: 32 lanes per column up to 160 levels (at most 5 levels per
 // lane; more push the kernel to one wavefront per SIMD).
 void default_shape(int precision, int nk, p3d::Shape* sh) {
-  (void)precision;
   sh->cb = 32;
   sh->nt = 256;
   sh->wpe = 2;
-  sh->lpc = nk <= 48 ? 16 : (nk <= 32 * 5 ? 32 : 64);
+  // Sedimentation lanes per column.  FP64 holds more registers per level, so it
+  // moves to 32 lanes (at most 3 levels per lane at 72 levels) sooner; the mixed
+  // form still runs two wavefronts per SIMD with 5 levels per lane (measured at
+  // 21600 x 72: FP64 1115 us at 32 lanes, 1573 at 16; mixed 638 at 16, 689 at 32).
+  const int nk16 = precision == P3X_MIXED ? 80 : 48;
+  sh->lpc = nk <= nk16 ? 16 : (nk <= 32 * 5 ? 32 : 64);
 }
 
 // Offsets (in elements) of the tables in one device allocation; each is a

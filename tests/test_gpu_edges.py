@@ -1,0 +1,117 @@
+"""Edge shapes of p3x_main against the oracle: one column, one to three levels
+(cut from the lower troposphere of a 72-level column so that cloud, rain and
+ice are all present), the largest supported level count (512, 64 lanes of 8
+levels in sedimentation), a padded leading dimension (ld > ncol; the padding
+must stay untouched), and an empty column set (nj = 0, nothing launched)."""
+import numpy as np
+import pytest
+
+import oracle
+from scream_amd import p3x
+from scream_amd.inputs import make_columns
+
+from p3_compare import FLOAT_FIELDS, field_errors, run_gpu
+
+pytestmark = pytest.mark.gpu
+
+RTOL_FP64 = 1e-10
+MATH = "portable"
+
+
+@pytest.fixture(scope="module")
+def ctx(host_tables):
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    c = p3x.Context(host_tables)
+    yield c
+    c.close()
+
+
+def _levels(inp, k0, k1):
+    return {k: np.ascontiguousarray(v[k0:k1]) for k, v in inp.items()}
+
+
+def _check(got, gdbg, want, wdbg, label):
+    errs = field_errors(got, want)
+    bad = {k: v for k, v in errs.items() if not v <= RTOL_FP64}
+    assert not bad, f"[{label}] fields beyond {RTOL_FP64}: {bad}"
+    for k in ("col_status", "sed_substeps", "ice_table_idx"):
+        np.testing.assert_array_equal(gdbg[k], wdbg[k], err_msg=f"{label} {k}")
+
+
+@pytest.mark.parametrize("ncol,k0,k1,dt", [(1, 66, 67, 1800.0), (1, 0, 1, 300.0), (2, 63, 65, 300.0),
+                                           (7, 62, 65, 1800.0), (33, 56, 72, 1800.0)])
+def test_few_levels(ctx, oracle_tables, ncol, k0, k1, dt):
+    inp = _levels(make_columns(ncol, 72, seed=50 + ncol, dry_every=7, evap_every=9), k0, k1)
+    print(f"few levels: {ncol} columns, levels {k0}..{k1 - 1}, dt {dt}", flush=True)
+    want, wdbg, _ = oracle.run_oracle(inp, oracle_tables, dt, True, False, math=MATH)
+    got, gdbg, _ = run_gpu(ctx, inp, dt, True, False)
+    _check(got, gdbg, want, wdbg, f"{ncol}x{k1 - k0}")
+
+
+def test_most_levels(ctx, oracle_tables):
+    ncol, nlev, dt = 65, 512, 100.0
+    inp = make_columns(ncol, nlev, seed=61, dry_every=7, evap_every=9)
+    print(f"most levels: {ncol}x{nlev}", flush=True)
+    want, wdbg, _ = oracle.run_oracle(inp, oracle_tables, dt, True, False, math=MATH)
+    got, gdbg, _ = run_gpu(ctx, inp, dt, True, False)
+    _check(got, gdbg, want, wdbg, f"{ncol}x{nlev}")
+
+
+def test_too_many_levels_is_an_argument_error(ctx):
+    import torch
+    dev = torch.device("cuda:0")
+    nlev, ncol = 513, 4
+    f = {k: torch.zeros((nlev + 1, ncol), dtype=torch.float64, device=dev)
+         for k in p3x.STATE + p3x.INPUTS + p3x.OUTS + p3x.HIST}
+    with pytest.raises(p3x.P3Error):
+        ctx.main(f, ncol, nlev, 300.0)
+
+
+def test_padded_leading_dimension(ctx, oracle_tables):
+    """Level-major arrays with ld = ncol + 13: the first ncol entries of each row
+    are the columns; the 13 padding entries hold a sentinel that must survive."""
+    import torch
+    dev = torch.device("cuda:0")
+    ncol, nlev, ld, dt = 300, 72, 313, 1800.0
+    inp = make_columns(ncol, nlev, seed=71, dry_every=7, evap_every=9)
+    sentinel = -12345.5
+    d = {}
+    for k in p3x.STATE + p3x.INPUTS:
+        a = np.full((nlev, ld), sentinel)
+        a[:, :ncol] = inp[k]
+        d[k] = torch.from_numpy(a).to(dev)
+    for k in p3x.OUTS + p3x.HIST:
+        if k in ("precip_liq_surf", "precip_ice_surf"):
+            d[k] = torch.full((ncol,), np.nan, dtype=torch.float64, device=dev)
+            continue
+        rows = nlev + 1 if k in ("precip_liq_flux", "precip_ice_flux") else nlev
+        a = np.full((rows, ld), sentinel)
+        a[:, :ncol] = 0.0 if k in p3x.HIST else np.nan
+        d[k] = torch.from_numpy(a).to(dev)
+    print(f"padded: {ncol}x{nlev} ld {ld}", flush=True)
+    ctx.main(d, ncol, nlev, dt, True, False, ld=ld, stream=torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    want, _, _ = oracle.run_oracle(inp, oracle_tables, dt, True, False, math=MATH)
+    got = {}
+    for k, v in d.items():
+        a = v.cpu().numpy()
+        if a.ndim == 2:
+            np.testing.assert_array_equal(a[:, ncol:], sentinel, err_msg=f"padding of {k}")
+            a = np.ascontiguousarray(a[:, :ncol])
+        got[k] = a
+    errs = field_errors(got, want)
+    bad = {k: v for k, v in errs.items() if not v <= RTOL_FP64}
+    assert not bad, bad
+
+
+def test_no_columns(ctx):
+    import torch
+    dev = torch.device("cuda:0")
+    nlev = 72
+    f = {k: torch.full((nlev + 1, 1), 7.0, dtype=torch.float64, device=dev)
+         for k in p3x.STATE + p3x.INPUTS + p3x.OUTS + p3x.HIST}
+    ctx.main(f, 0, nlev, 300.0)
+    torch.cuda.synchronize()
+    for k, v in f.items():
+        assert bool((v == 7.0).all()), k
