@@ -279,27 +279,43 @@ __device__ __forceinline__ void sediment_run(const KParams& p, int i, int j, int
   nsub_out = nsub;
 }
 
-// One species: find_top (P3I/p3_find_impl.hpp:14-50) over the group's
-// column in the standard mapping (lane j: levels j*L .. j*L+L-1), then the
-// substeps on the fewest levels per lane that cover [k_top, nlev).  Every lane
-// of the wavefront calls this (run = its column takes part); k_top_out = nlev
-// when the species is absent.
-template <int SP, class R, int LPC, int L>
-__device__ __forceinline__ void sediment(const KParams& p, int i, int j, bool run, double& prt, int& nsub_out,
-                                         int& k_top_out, unsigned& err) {
+// find_top (P3I/p3_find_impl.hpp:14-50) of the three species over the group's
+// column in the standard mapping (lane j: levels j*L .. j*L+L-1), all loads in
+// one pass; top[s] = nlev when species s is absent (or the column does not run).
+template <int LPC, int L>
+__device__ __forceinline__ void find_tops(const KParams& p, int i, int j, bool run, int (&top)[3]) {
   const int nlev = p.nlev;
   const size_t ld = (size_t)p.ld;
-  const double* q0 = SP == 0 ? p.qc : (SP == 1 ? p.qr : p.qi);
-  int my_top = nlev;
+  int my[3] = {nlev, nlev, nlev};
   if (run) {
+    double q[3][L];
 #pragma unroll
     for (int t = 0; t < L; ++t) {
       const int k = j * L + t;
-      if (k < nlev && q0[(size_t)k * ld + i] >= QSMALL && k < my_top) my_top = k;
+      const size_t x = (size_t)(k < nlev ? k : 0) * ld + i;
+      q[0][t] = p.qc[x];
+      q[1][t] = p.qr[x];
+      q[2][t] = p.qi[x];
+    }
+#pragma unroll
+    for (int t = L - 1; t >= 0; --t) {
+      const int k = j * L + t;
+#pragma unroll
+      for (int sp = 0; sp < 3; ++sp)
+        if (k < nlev && q[sp][t] >= QSMALL) my[sp] = k;
     }
   }
-  const int k_top = grp_min<LPC>(my_top);
-  k_top_out = k_top;
+#pragma unroll
+  for (int sp = 0; sp < 3; ++sp) top[sp] = grp_min<LPC>(my[sp]);
+}
+
+// One species' substeps on the fewest levels per lane that cover [k_top, nlev).
+// Every lane of the wavefront calls this (k_top = nlev for a lane whose column
+// does not run or has none of the species).
+template <int SP, class R, int LPC, int L>
+__device__ __forceinline__ void sediment(const KParams& p, int i, int j, int k_top, double& prt, int& nsub_out,
+                                         unsigned& err) {
+  const int nlev = p.nlev;
   prt = 0;
   nsub_out = 0;
   // levels per lane for [k_top, nlev), the largest over the wavefront's
@@ -477,10 +493,12 @@ __device__ __forceinline__ void sed_block(const KParams& p, int bid) {
   const int status = live ? p.colstat[i] : 0;
   const bool run = live && status == kStatusFull;
   double prt_c, prt_r, prt_i;
-  int ns_c, ns_r, ns_i, top_c, top_r, top_i;
-  sediment<0, R, LPC, L>(p, i, j, run, prt_c, ns_c, top_c, err);
-  sediment<1, R, LPC, L>(p, i, j, run, prt_r, ns_r, top_r, err);
-  sediment<2, R, LPC, L>(p, i, j, run, prt_i, ns_i, top_i, err);
+  int ns_c, ns_r, ns_i, top[3];
+  find_tops<LPC, L>(p, i, j, run, top);
+  const int top_c = top[0], top_r = top[1], top_i = top[2];
+  sediment<0, R, LPC, L>(p, i, j, top_c, prt_c, ns_c, err);
+  sediment<1, R, LPC, L>(p, i, j, top_r, prt_r, ns_r, err);
+  sediment<2, R, LPC, L>(p, i, j, top_i, prt_i, ns_i, err);
   if (live) {
     const double inv_dt = 1 / p.dt;
     const bool pres_c = top_c < nlev, pres_r = top_r < nlev, pres_i = top_i < nlev;
