@@ -381,6 +381,27 @@ def copy_probe():
     print(json.dumps({"copy_probe_bytes": COPY_PROBE_BYTES}))
 
 
+def copy_bandwidth(reps=10):
+    """Measured streaming bandwidth of this GPU: a 1 GiB device-to-device copy (read
+    + write = 2 GiB moved per copy), timed with HIP events, best of reps."""
+    import torch
+    n = COPY_PROBE_BYTES // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda:0")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    best = float("inf")
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        dst.copy_(src)
+        e1.record()
+        e1.synchronize()
+        best = min(best, e0.elapsed_time(e1) / 1000.0)
+    del src, dst
+    torch.cuda.empty_cache()
+    return 2 * COPY_PROBE_BYTES / best / 1e9
+
+
 def _pmc_means(directory, counter, prefix="*"):
     """Mean counter value per dispatch, by kernel name, from the rocprofv3 csv output
     files named <prefix>_counter_collection.csv under directory."""
@@ -462,6 +483,7 @@ def main():
         copy_probe()
         return
     ctx = p3x.Context()
+    copy_gbs = copy_bandwidth()
 
     ncol, nlev, dt = 21600, 72, 1800.0
     w = Workload(ncol, nlev, seed=1)
@@ -480,6 +502,8 @@ def main():
                    "predictNc": True, "prescribedCCN": False, "kernel_us_per_step": kern_us,
                    "part12_kernel_us": k12_us, "sed_kernel_us": sed_us,
                    "step_hbm_frac": BYTES_PER_COLUMN_LEVEL * colev / (kern_us * 1e-6) / 1e9 / HBM_PEAK_GBS,
+                   "copy_GBps": copy_gbs,
+                   "step_frac_of_copy": BYTES_PER_COLUMN_LEVEL * colev / (kern_us * 1e-6) / 1e9 / copy_gbs,
                    "step_traffic_bytes": traffic_for("g64_fp64")},
         # the dominant kernel (sedimentation + part 3): algorithmic bytes per launch over its
         # event-timed duration, and its measured HBM bytes per launch
