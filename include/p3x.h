@@ -29,7 +29,10 @@
  *
  * Thread safety: one context may be used by one host thread at a time.
  * Every call is enqueued on the stream given in p3x_launch (default stream if
- * NULL) and, unless p3x_launch.async is set, is complete on return.
+ * NULL) and, unless p3x_launch.async is set, is complete on return.  The
+ * p3x_main calls of one context share its scratch: a call on another stream
+ * than the previous call's is made to wait on the device for that call, so
+ * asynchronous calls on several streams run one after another.
  */
 #ifndef P3X_H
 #define P3X_H

@@ -246,6 +246,8 @@ extern "C" void p3x_finalize(p3x_context* c) {
 extern "C" int p3x_sync(p3x_context* c) {
   if (!c) return P3X_ERR_ARG;
   hipError_t e = hipStreamSynchronize(c->last_stream);
+  if (e == hipSuccess && c->main_launched && c->main_stream != c->last_stream)
+    e = hipStreamSynchronize(c->main_stream);
   if (e != hipSuccess) return P3X_ERR_DEVICE;
   unsigned err = 0;
   e = hipMemcpy(&err, c->d_err, sizeof(unsigned), hipMemcpyDeviceToHost);
@@ -289,7 +291,7 @@ int p3x_detail::main_impl(p3x_context* c, const p3x_runtime* runtime, const p3x_
 
   const size_t need = scratch_bytes_for(nk, ld);
   if (need > c->scratch_bytes) {
-    hipError_t e = hipStreamSynchronize(c->last_stream);
+    hipError_t e = hipStreamSynchronize(c->main_stream);
     if (e == hipSuccess && c->d_scratch) e = hipFree(c->d_scratch);
     c->d_scratch = nullptr;
     c->scratch_bytes = 0;
@@ -303,7 +305,14 @@ int p3x_detail::main_impl(p3x_context* c, const p3x_runtime* runtime, const p3x_
               o.ice_table_idx, c->d_scratch, &k);
   if (amble) k.amble = *amble;
 
-  hipError_t e = hipEventRecord(c->ev0, s);
+  // Every call on this context shares its scratch: a call on another stream than
+  // the previous one waits on the device for the previous one to end.
+  hipError_t e = hipSuccess;
+  if (c->main_launched && c->main_stream != s) {
+    e = hipStreamWaitEvent(s, c->ev1, 0);
+    if (e != hipSuccess) return P3X_ERR_DEVICE;
+  }
+  e = hipEventRecord(c->ev0, s);
   if (e != hipSuccess) return P3X_ERR_DEVICE;
   e = o.precision == P3X_FP64 ? p3d::launch_p3_fp64(k, sh, s, c->ev_mid) : p3d::launch_p3_mixed(k, sh, s, c->ev_mid);
   if (e == hipErrorInvalidValue) return P3X_ERR_ARG;
@@ -311,6 +320,8 @@ int p3x_detail::main_impl(p3x_context* c, const p3x_runtime* runtime, const p3x_
   e = hipEventRecord(c->ev1, s);
   if (e != hipSuccess) return P3X_ERR_DEVICE;
   c->last_stream = s;
+  c->main_stream = s;
+  c->main_launched = 1;
   if (o.async) return P3X_OK;
   const int st = p3x_sync(c);
   if (elapsed_us) {

@@ -21,6 +21,10 @@ struct p3x_context {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipEvent_t ev_mid = nullptr;  // between the part 1 + 2 and the sedimentation kernels
   hipStream_t last_stream = nullptr;
+  // the stream of the last p3x_main on this context (the last user of d_scratch);
+  // ev1 marks the end of its work once main_launched is set
+  hipStream_t main_stream = nullptr;
+  int main_launched = 0;
   int pending_timing = 0;
 };
 

@@ -115,3 +115,27 @@ def test_no_columns(ctx):
     torch.cuda.synchronize()
     for k, v in f.items():
         assert bool((v == 7.0).all()), k
+
+
+def test_async_calls_on_two_streams_share_scratch_safely(ctx, oracle_tables):
+    """Two asynchronous p3x_main calls of one context on two streams (different
+    column sets, same scratch) give the results each gives alone."""
+    import torch
+    from p3_compare import device_fields
+    inp_a = make_columns(700, 72, seed=81, dry_every=7, evap_every=9)
+    inp_b = make_columns(500, 128, seed=82, dry_every=7, evap_every=9)
+    want_a, _, _ = oracle.run_oracle(inp_a, oracle_tables, 1800.0, True, False, math=MATH)
+    want_b, _, _ = oracle.run_oracle(inp_b, oracle_tables, 300.0, True, False, math=MATH)
+    da, dbg_a = device_fields(inp_a)
+    db, dbg_b = device_fields(inp_b)
+    sa, sb = torch.cuda.Stream(), torch.cuda.Stream()
+    torch.cuda.synchronize()
+    print("two streams: 700x72 then 500x128, async", flush=True)
+    ctx.main(da, 700, 72, 1800.0, True, False, stream=sa.cuda_stream, async_=True, debug=dbg_a)
+    ctx.main(db, 500, 128, 300.0, True, False, stream=sb.cuda_stream, async_=True, debug=dbg_b)
+    assert ctx.sync() == 0
+    torch.cuda.synchronize()
+    for got, want in ((da, want_a), (db, want_b)):
+        errs = field_errors({k: v.cpu().numpy() for k, v in got.items()}, want)
+        bad = {k: v for k, v in errs.items() if not v <= RTOL_FP64}
+        assert not bad, bad
