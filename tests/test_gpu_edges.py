@@ -139,3 +139,26 @@ def test_async_calls_on_two_streams_share_scratch_safely(ctx, oracle_tables):
         errs = field_errors({k: v.cpu().numpy() for k, v in got.items()}, want)
         bad = {k: v for k, v in errs.items() if not v <= RTOL_FP64}
         assert not bad, bad
+
+
+@pytest.mark.parametrize("bad", [-5.0, float("nan")])
+def test_bad_temperature_is_reported_not_fatal(ctx, oracle_tables, bad):
+    """A level with T <= 0 or NaN (the reference's EKAT_KERNEL_REQUIRE abort in
+    check_temperature, share/physics_saturation_impl.hpp:30-35) makes the call
+    return P3X_ERR_TEMPERATURE; the context stays usable and the next call on
+    good columns is exact."""
+    import torch
+    from p3_compare import device_fields
+    inp = make_columns(64, 72, seed=91)
+    inp["th"] = inp["th"].copy()
+    inp["th"][40, 17] = bad
+    d, dbg = device_fields(inp)
+    print(f"bad temperature {bad}", flush=True)
+    with pytest.raises(p3x.P3Error) as e:
+        ctx.main(d, 64, 72, 300.0, True, False, stream=torch.cuda.current_stream().cuda_stream, debug=dbg)
+    assert e.value.status == p3x.P3X_ERR_TEMPERATURE
+    good = make_columns(64, 72, seed=92, dry_every=7, evap_every=9)
+    want, _, _ = oracle.run_oracle(good, oracle_tables, 300.0, True, False, math=MATH)
+    got, _, _ = run_gpu(ctx, good, 300.0, True, False)
+    errs = field_errors(got, want)
+    assert max(errs.values()) <= RTOL_FP64, errs
