@@ -244,15 +244,18 @@ __device__ __forceinline__ void part2_tile(const KParams& p, int tile) {
   auto scr = [&](int f) -> double { return p.scratch[((size_t)f * nlev + k) * ld + i]; };
   unsigned err = 0;
   bool done = false;
-  // bit 1 only: other levels of the column may already have set bit 2
-  if (p.colstat[i] & kStatusPart1) {
+  // The column status and the level's skip-test inputs are loaded together (one
+  // memory round trip for the decision); bit 1 only: other levels of the column
+  // may already have set bit 2.
+  {
     Lvl s;
+    const int cstat = p.colstat[i];
     s.qc = p.qc[x]; s.qr = p.qr[x]; s.qi = p.qi[x];
     s.T_atm = scr(SC_T);
     s.qv_supersat_i = scr(SC_SUPI);
     const bool skip_all = s.qc < QSMALL && s.qr < QSMALL && s.qi < QSMALL && s.T_atm < T_zerodegc &&
                           s.qv_supersat_i < -0.05;
-    if (!skip_all) {
+    if ((cstat & kStatusPart1) && !skip_all) {
       s.nc = p.nc[x]; s.nr = p.nr[x]; s.qm = p.qm[x]; s.ni = p.ni[x]; s.bm = p.bm[x];
       s.qv = p.qv[x]; s.th = p.th[x];
       s.pres = p.pres[x];
