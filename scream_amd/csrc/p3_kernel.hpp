@@ -129,7 +129,7 @@ __device__ __forceinline__ void part1_block(const KParams& p, int bid, int* s_i)
         qc = p.qc[x] * dp / dp_dry; nc = p.nc[x] * dp / dp_dry; qr = p.qr[x] * dp / dp_dry;
         nr = p.nr[x] * dp / dp_dry; qi = p.qi[x] * dp / dp_dry; ni = p.ni[x] * dp / dp_dry;
         qm = p.qm[x] * dp / dp_dry; bm = p.bm[x] * dp / dp_dry;
-        qv = vmax(p.qv[x] * dp / dp_dry, 0.0);
+        qv = p.qv[x] * dp / dp_dry;
         a.qv_prev[x] = a.qv_prev[x] * dp / dp_dry;
         inv_exner = 1.0 / exner_function(pmid);
         a.inv_exner[x] = inv_exner;
@@ -147,12 +147,16 @@ __device__ __forceinline__ void part1_block(const KParams& p, int bid, int* s_i)
       } else {
         inv_exner = p.inv_exner[x];
         th = p.th[x];
-        qv = vmax(p.qv[x], 0.0);
+        qv = p.qv[x];
         qc = p.qc[x]; nc = p.nc[x]; qr = p.qr[x]; nr = p.nr[x];
         qi = p.qi[x]; ni = p.ni[x]; qm = p.qm[x]; bm = p.bm[x];
         dz = p.dz[x];
         cfl = p.prescribedCCN ? p.cld_frac_l[x] : 1.0;
       }
+      // the state as it is in memory: a field part 1 leaves bit for bit unchanged
+      // is not stored again (with the pre-amble every field changes to dry)
+      const double in[10] = {qc, nc, qr, nr, qi, ni, qm, bm, qv, th};
+      qv = vmax(qv, 0.0);
       const double exner = 1 / inv_exner;
       const double pres = p.pres[x];
       double T_atm = th * exner;
@@ -202,8 +206,13 @@ __device__ __forceinline__ void part1_block(const KParams& p, int bid, int* s_i)
       }
       T_atm = th * exner;
 
-      p.qc[x] = qc; p.nc[x] = nc; p.qr[x] = qr; p.nr[x] = nr; p.qi[x] = qi;
-      p.ni[x] = ni; p.qm[x] = qm; p.bm[x] = bm; p.qv[x] = qv; p.th[x] = th;
+      {
+        double* const f[10] = {p.qc, p.nc, p.qr, p.nr, p.qi, p.ni, p.qm, p.bm, p.qv, p.th};
+        const double out[10] = {qc, nc, qr, nr, qi, ni, qm, bm, qv, th};
+#pragma unroll
+        for (int q = 0; q < 10; ++q)
+          if (p.amble.on || __double_as_longlong(out[q]) != __double_as_longlong(in[q])) f[q][x] = out[q];
+      }
       scr(SC_T, k) = T_atm;
       scr(SC_QVSL, k) = qv_sat_l;
       scr(SC_QVSI, k) = qv_sat_i;
