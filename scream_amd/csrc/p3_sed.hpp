@@ -347,6 +347,9 @@ __device__ __forceinline__ void part3_level(const KParams& p, int k, int i, bool
     qc = p.qc[x]; nc = p.nc[x]; qr = p.qr[x]; nr = p.nr[x]; qi = p.qi[x]; ni = p.ni[x];
     qm = p.qm[x]; bm = p.bm[x]; qv = p.qv[x]; th = p.th[x];
   }
+  // the state as it is in memory: without the post-amble, a field that part 3
+  // leaves bit for bit unchanged is not stored again
+  const double in[10] = {qc, nc, qr, nr, qi, ni, qm, bm, qv, th};
   if (active) {
     const double inv_exner = p.inv_exner[x];
     if (rt.do_ice_production) {
@@ -456,8 +459,11 @@ __device__ __forceinline__ void part3_level(const KParams& p, int k, int i, bool
     reff_i *= 1e6;
     reff_r *= 1e6;
   } else if (active) {
-    p.qc[x] = qc; p.nc[x] = nc; p.qr[x] = qr; p.nr[x] = nr; p.qi[x] = qi;
-    p.ni[x] = ni; p.qm[x] = qm; p.bm[x] = bm; p.qv[x] = qv; p.th[x] = th;
+    double* const f[10] = {p.qc, p.nc, p.qr, p.nr, p.qi, p.ni, p.qm, p.bm, p.qv, p.th};
+    const double out[10] = {qc, nc, qr, nr, qi, ni, qm, bm, qv, th};
+#pragma unroll
+    for (int q = 0; q < 10; ++q)
+      if (__double_as_longlong(out[q]) != __double_as_longlong(in[q])) f[q][x] = out[q];
   }
   p.diag_eff_radius_qc[x] = reff_c;
   p.diag_eff_radius_qi[x] = reff_i;
