@@ -265,13 +265,18 @@ __device__ __forceinline__ void sediment_run(const KParams& p, int i, int j, int
     nsub++;
     first = false;
   }
+  // The substeps changed levels k_top .. k_bot only (k_bot has moved down one
+  // level after each substep, past the last one's lowest updated level): the
+  // levels below keep their loaded values and are not stored again.
 #pragma unroll
   for (int t = 0; t < L; ++t) {
     const int k = kbase + t;
     if (k < nlev) {
       const size_t x = (size_t)k * ld + i;
+      if (k <= k_bot) {
 #pragma unroll
-      for (int f = 0; f < NF; ++f) fld[f][x] = q[f][t];
+        for (int f = 0; f < NF; ++f) fld[f][x] = q[f][t];
+      }
       if (SP == 1) p.precip_liq_flux[x + ld] = pflux[t];  // rain_sed :140-153, interface k+1
     }
   }
