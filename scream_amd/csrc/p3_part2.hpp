@@ -91,14 +91,31 @@ __device__ bool part2_level(const Runtime& rt, const Tables& tb, double dt, bool
       s.bm = double(bm_incld) * cld_frac_i;
       const TabIce<R> ti = lookup_ice<R>(qi_incld, ni_incld, qm_incld, rhop);
       const TabRain<R> tr = lookup_rain<R>(qr_incld, nr_incld);
-      // ice quantities 1,2,3,4 then 6,7,9 of the part 2 records
+      // ice quantities 1,2,3,4 then 6,7,9 of the part 2 records (in FP64 two at a
+      // time: 8 corners of 2 values live at once instead of 4, fewer spills)
       R v[4];
-      apply_table_ice_rec<4, 8, R>(TabOf<R>::get(tb).ice_p2, ti, v, 0);
+      if constexpr (sizeof(R) == 8) {
+        R a[2];
+        apply_table_ice_rec<2, 8, R>(TabOf<R>::get(tb).ice_p2, ti, a, 0);
+        v[0] = a[0]; v[1] = a[1];
+        apply_table_ice_rec<2, 8, R>(TabOf<R>::get(tb).ice_p2, ti, a, 2);
+        v[2] = a[0]; v[3] = a[1];
+      } else {
+        apply_table_ice_rec<4, 8, R>(TabOf<R>::get(tb).ice_p2, ti, v, 0);
+      }
       tv_qi_fallspd = v[0];
       tv_ni_self_collect = v[1];
       tv_qc2qi_collect = v[2];
       tv_qi2qr_melting = v[3];
-      apply_table_ice_rec<4, 8, R>(TabOf<R>::get(tb).ice_p2, ti, v, 4);
+      if constexpr (sizeof(R) == 8) {
+        R a[2];
+        apply_table_ice_rec<2, 8, R>(TabOf<R>::get(tb).ice_p2, ti, a, 4);
+        v[0] = a[0]; v[1] = a[1];
+        apply_table_ice_rec<2, 8, R>(TabOf<R>::get(tb).ice_p2, ti, a, 6);
+        v[2] = a[0];
+      } else {
+        apply_table_ice_rec<4, 8, R>(TabOf<R>::get(tb).ice_p2, ti, v, 4);
+      }
       tv_lammax = v[0];
       tv_lammin = v[1];
       tv_vent_melt = v[2];

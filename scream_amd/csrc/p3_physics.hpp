@@ -241,31 +241,32 @@ __device__ __forceinline__ void apply_table_coll2(const R* __restrict__ tab, con
   const R w3 = (tr.dum3 - R(j) - R(1));
   const R w4 = (ti.dum4 - R(ii) - R(1));
   const R w5 = (ti.dum5 - R(jj) - R(1));
-  R c[16][2];
+  // one rime-density plane (8 corners) at a time: the same arithmetic as all 16
+  // corners at once, with half the values live
+  auto plane = [&](int a, R (&tmp)[2]) {
+    R c[8][2];
 #pragma unroll
-  for (int n = 0; n < 16; ++n) {
-    // corner bits: 8 = density, 4 = rime, 2 = size, 1 = rain size
-    load_rec<2>(tab + coll_off(jj + (n >> 3), ii + ((n >> 2) & 1), i + ((n >> 1) & 1), j + (n & 1), 0), c[n]);
-  }
+    for (int n = 0; n < 8; ++n) {
+      // corner bits: 4 = rime, 2 = size, 1 = rain size
+      load_rec<2>(tab + coll_off(jj + a, ii + (n >> 2), i + ((n >> 1) & 1), j + (n & 1), 0), c[n]);
+    }
 #pragma unroll
-  for (int q = 0; q < 2; ++q) {
-    auto T = [&](int a, int b, int x, int y) { return c[a * 8 + b * 4 + x * 2 + y][q]; };
-    R dproc1 = T(0, 0, 0, 0) + w1 * (T(0, 0, 1, 0) - T(0, 0, 0, 0));
-    R dproc2 = T(0, 0, 0, 1) + w1 * (T(0, 0, 1, 1) - T(0, 0, 0, 1));
-    R iproc1 = dproc1 + w3 * (dproc2 - dproc1);
-    dproc1 = T(0, 1, 0, 0) + w1 * (T(0, 1, 1, 0) - T(0, 1, 0, 0));
-    dproc2 = T(0, 1, 0, 1) + w1 * (T(0, 1, 1, 1) - T(0, 1, 0, 1));
-    R gproc1 = dproc1 + w3 * (dproc2 - dproc1);
-    const R tmp1 = iproc1 + w4 * (gproc1 - iproc1);
-    dproc1 = T(1, 0, 0, 0) + w1 * (T(1, 0, 1, 0) - T(1, 0, 0, 0));
-    dproc2 = T(1, 0, 0, 1) + w1 * (T(1, 0, 1, 1) - T(1, 0, 0, 1));
-    iproc1 = dproc1 + w3 * (dproc2 - dproc1);
-    dproc1 = T(1, 1, 0, 0) + w1 * (T(1, 1, 1, 0) - T(1, 1, 0, 0));
-    dproc2 = T(1, 1, 0, 1) + w1 * (T(1, 1, 1, 1) - T(1, 1, 0, 1));
-    gproc1 = dproc1 + w3 * (dproc2 - dproc1);
-    const R tmp2 = iproc1 + w4 * (gproc1 - iproc1);
-    out[q] = tmp1 + w5 * (tmp2 - tmp1);
-  }
+    for (int q = 0; q < 2; ++q) {
+      auto T = [&](int b, int x, int y) { return c[b * 4 + x * 2 + y][q]; };
+      R dproc1 = T(0, 0, 0) + w1 * (T(0, 1, 0) - T(0, 0, 0));
+      R dproc2 = T(0, 0, 1) + w1 * (T(0, 1, 1) - T(0, 0, 1));
+      const R iproc1 = dproc1 + w3 * (dproc2 - dproc1);
+      dproc1 = T(1, 0, 0) + w1 * (T(1, 1, 0) - T(1, 0, 0));
+      dproc2 = T(1, 0, 1) + w1 * (T(1, 1, 1) - T(1, 0, 1));
+      const R gproc1 = dproc1 + w3 * (dproc2 - dproc1);
+      tmp[q] = iproc1 + w4 * (gproc1 - iproc1);
+    }
+  };
+  R tmp1[2], tmp2[2];
+  plane(0, tmp1);
+  plane(1, tmp2);
+#pragma unroll
+  for (int q = 0; q < 2; ++q) out[q] = tmp1[q] + w5 * (tmp2[q] - tmp1[q]);
 }
 
 // lookup (Table3), P3I/p3_table3_impl.hpp:14-62
