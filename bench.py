@@ -115,8 +115,9 @@ def time_steps(ctx, w, dt, precision, steps, warmup):
 
 
 def kernel_split(ctx, w, dt, precision, steps):
-    """Mean device microseconds of the two kernels of a step, (part 1 + 2, sedimentation + part 3),
-    from HIP events the library records on the kernels' stream around and between them."""
+    """Mean device microseconds of a step in two intervals, (the part 1 and part 2 kernels, the
+    sedimentation + part 3 kernel), from HIP events the library records on the kernels'
+    stream around and between them."""
     import torch
     sp = torch.cuda.current_stream().cuda_stream
     a, b = [], []

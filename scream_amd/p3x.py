@@ -353,7 +353,8 @@ class Context:
         return self._lib.p3x_sync(self.handle)
 
     def last_kernel_times(self) -> tuple[float, float]:
-        """Device microseconds of the last main()'s two kernels: (part 1 + 2, sedimentation + part 3)."""
+        """Device microseconds of the last main() in two intervals: (the part 1 and part 2
+        kernels, the sedimentation + part 3 kernel)."""
         a, b = C.c_double(), C.c_double()
         st = self._lib.p3x_last_kernel_times(self.handle, C.byref(a), C.byref(b))
         if st != P3X_OK:
