@@ -421,9 +421,10 @@ def _pmc_means(directory, counter, prefix="*"):
     return {k: v[0] / v[1] for k, v in sums.items()}
 
 
-def summarize_pmc(fetch_dir, write_dir):
+def summarize_pmc(fetch_dir, write_dir, out_path=None):
     """HBM bytes per step of the headline workload from FETCH_SIZE and WRITE_SIZE (KiB per
-    dispatch), each scaled by the bytes-per-KiB the same counter reports for the copy probe."""
+    dispatch), each scaled by the bytes-per-KiB the same counter reports for the copy probe;
+    written to out_path (default profiles/traffic.json)."""
     fetch, write = _pmc_means(fetch_dir, "FETCH_SIZE", "bench"), _pmc_means(write_dir, "WRITE_SIZE", "bench")
     pick = lambda d, key: {k: v for k, v in d.items() if key in k}
     copy_f = max(_pmc_means(fetch_dir, "FETCH_SIZE", "copy").values())
@@ -445,16 +446,16 @@ def summarize_pmc(fetch_dir, write_dir):
                        "write_bytes_per_KiB": scale_w},
         "kernels": kern,
         "bytes_per_step": sum(v["bytes"] for v in kern.values())}}
-    path = os.path.join(ROOT, "profiles", "traffic.json")
+    path = out_path or os.path.join(ROOT, "profiles", "traffic.json")
     with open(path, "w") as fh:
         json.dump(out, fh, indent=1)
     print(json.dumps(out))
 
 
-def traffic_for(workload, kernel=None):
+def traffic_for(workload, kernel=None, path=None):
     """HBM bytes per step (or per launch of `kernel`) measured by rocprofv3 counters
-    (profiles/traffic.json), or None."""
-    path = os.path.join(ROOT, "profiles", "traffic.json")
+    (profiles/traffic.json unless path is given), or None."""
+    path = path or os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
