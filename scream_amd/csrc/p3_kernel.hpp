@@ -279,11 +279,20 @@ __device__ __forceinline__ void part2_tile(const KParams& p, int tile) {
       s.inv_exner = p.inv_exner[x];
       s.qv_prev = p.qv_prev[x]; s.t_prev = p.t_prev[x];
       s.ni_activated = p.ni_activated[x];
+      const double in[10] = {s.qc, s.nc, s.qr, s.nr, s.qi, s.ni, s.qm, s.bm, s.qv, s.th};
       // hydrometeorsPresent after part 2 (P3I/p3_main_impl.hpp:255-260): a column flag
       if (part2_level<R>(p.rt, p.tab, p.dt, p.predictNc, p.prescribedCCN, p.e_melt, s, err))
         atomicOr(&p.colstat[i], kStatusPart2);
-      p.qc[x] = s.qc; p.nc[x] = s.nc; p.qr[x] = s.qr; p.nr[x] = s.nr; p.qi[x] = s.qi;
-      p.ni[x] = s.ni; p.qm[x] = s.qm; p.bm[x] = s.bm; p.qv[x] = s.qv; p.th[x] = s.th;
+      {
+        // mixed form: a state field whose bits part 2 left unchanged is not stored
+        // again (in FP64 the ten held originals cost more registers than the
+        // stores they save)
+        double* const f[10] = {p.qc, p.nc, p.qr, p.nr, p.qi, p.ni, p.qm, p.bm, p.qv, p.th};
+        const double out[10] = {s.qc, s.nc, s.qr, s.nr, s.qi, s.ni, s.qm, s.bm, s.qv, s.th};
+#pragma unroll
+        for (int q = 0; q < 10; ++q)
+          if (sizeof(R) == 8 || __double_as_longlong(out[q]) != __double_as_longlong(in[q])) f[q][x] = out[q];
+      }
       p.qv2qi_depos_tend[x] = s.qv2qi_depos_tend;
       p.precip_total_tend[x] = s.precip_total_tend;
       p.nevapr[x] = s.nevapr;
