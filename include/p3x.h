@@ -177,6 +177,13 @@ int p3x_tables_load(const char* ice_table_file, p3x_table_data* out);
 /* Create a context on the current HIP device: uploads the tables. */
 int p3x_init(const p3x_table_data* tables, p3x_context** ctx);
 void p3x_finalize(p3x_context* ctx);
+/* Allocate the context's scratch for column sets up to ld columns (the
+ * leading dimension) by nk levels, as initialize_impl sizes the process's
+ * buffers once (eamxx_p3_process_interface.cpp:124-217, the
+ * requested_buffer_size_in_bytes / init_buffers pair): after it, p3x_main
+ * allocates nothing for shapes within it.  Without it, p3x_main allocates on
+ * the first call and whenever a larger shape arrives. */
+int p3x_reserve(p3x_context* ctx, int ld, int nk);
 
 /* One P3 step over nj columns x nk levels (p3_functions.hpp:1356-1369).
  * elapsed_us (optional) receives the device time of the step (HIP events),

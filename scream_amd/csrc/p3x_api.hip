@@ -243,6 +243,25 @@ extern "C" void p3x_finalize(p3x_context* c) {
   delete c;
 }
 
+// Scratch of at least need bytes; a larger allocation replaces the old one
+// once the last call that used it has ended.
+int p3x_detail::grow_scratch(p3x_context* c, size_t need) {
+  if (need <= c->scratch_bytes) return P3X_OK;
+  hipError_t e = hipStreamSynchronize(c->main_stream);
+  if (e == hipSuccess && c->d_scratch) e = hipFree(c->d_scratch);
+  c->d_scratch = nullptr;
+  c->scratch_bytes = 0;
+  if (e == hipSuccess) e = hipMalloc((void**)&c->d_scratch, need);
+  if (e != hipSuccess) return dev_fail(e);
+  c->scratch_bytes = need;
+  return P3X_OK;
+}
+
+extern "C" int p3x_reserve(p3x_context* c, int ld, int nk) {
+  if (!c || ld <= 0 || nk <= 0 || nk > 64 * p3d::kMaxLevelsPerLane) return P3X_ERR_ARG;
+  return grow_scratch(c, scratch_bytes_for(nk, ld));
+}
+
 extern "C" int p3x_sync(p3x_context* c) {
   if (!c) return P3X_ERR_ARG;
   hipError_t e = hipStreamSynchronize(c->last_stream);
@@ -289,15 +308,9 @@ int p3x_detail::main_impl(p3x_context* c, const p3x_runtime* runtime, const p3x_
   if (nj == 0) return P3X_OK;
   if (!fields_present(ps, di, dout, hist)) return P3X_ERR_ARG;
 
-  const size_t need = scratch_bytes_for(nk, ld);
-  if (need > c->scratch_bytes) {
-    hipError_t e = hipStreamSynchronize(c->main_stream);
-    if (e == hipSuccess && c->d_scratch) e = hipFree(c->d_scratch);
-    c->d_scratch = nullptr;
-    c->scratch_bytes = 0;
-    if (e == hipSuccess) e = hipMalloc((void**)&c->d_scratch, need);
-    if (e != hipSuccess) return dev_fail(e);
-    c->scratch_bytes = need;
+  {
+    const int st = grow_scratch(c, scratch_bytes_for(nk, ld));
+    if (st != P3X_OK) return st;
   }
 
   p3d::KParams k;

@@ -34,6 +34,8 @@ int dev_fail(hipError_t e);
 // Default launch shape per (precision, nlev).
 void default_shape(int precision, int nk, p3d::Shape* sh);
 // KParams of one column set; scratch points at SC_COUNT*nk*ld doubles followed by ld ints.
+// Grows the context's scratch to at least need bytes (P3X_OK or an error status).
+int grow_scratch(p3x_context* c, size_t need);
 // p3x_main with (amble != NULL) the P3Microphysics pre/post-amble fused into its kernels.
 int main_impl(p3x_context* c, const p3x_runtime* runtime, const p3x_prognostic_state* ps,
               const p3x_diagnostic_inputs* di, const p3x_diagnostic_outputs* dout,

@@ -168,6 +168,13 @@ extern "C" int p3x_process_create(p3x_context* c, const p3x_runtime* rt, int nco
   p->precip_ice_flux = m; m += iface;
   p->precip_liq_surf = m; m += ncol;
   p->precip_ice_surf = m;
+  // the context's scratch for this shape, once, as initialize_impl sizes its buffers
+  const int st = p3x_reserve(c, ncol, nlev);
+  if (st != P3X_OK) {
+    (void)hipFree(p->mem);
+    delete p;
+    return st;
+  }
   *out = p;
   return P3X_OK;
 }

@@ -36,7 +36,7 @@ HIST = ("liq_ice_exchange", "vap_liq_exchange", "vap_ice_exchange")
 ALL_FIELDS = STATE + INPUTS + OUTS + HIST
 
 # Every symbol include/p3x.h declares.
-EXPORTS = ("p3x_runtime_defaults", "p3x_launch_defaults", "p3x_tables_load", "p3x_init",
+EXPORTS = ("p3x_runtime_defaults", "p3x_launch_defaults", "p3x_tables_load", "p3x_init", "p3x_reserve",
            "p3x_finalize", "p3x_main", "p3x_sync", "p3x_last_kernel_times", "p3x_strerror",
            "p3x_batch_defaults",
            "p3x_batch_create", "p3x_batch_run", "p3x_batch_tune", "p3x_batch_destroy",
@@ -232,6 +232,8 @@ def load() -> C.CDLL:
         lib.p3x_cld_fraction.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_double,
                                          C.c_double] + [C.c_void_p] * 5
         lib.p3x_cld_fraction.restype = C.c_int
+        lib.p3x_reserve.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        lib.p3x_reserve.restype = C.c_int
         _lib = lib
     return _lib
 
@@ -351,6 +353,13 @@ class Context:
 
     def sync(self) -> int:
         return self._lib.p3x_sync(self.handle)
+
+    def reserve(self, ld: int, nk: int):
+        """p3x_reserve: scratch for column sets up to ld columns x nk levels, so that
+        main() allocates nothing for them."""
+        st = self._lib.p3x_reserve(self.handle, int(ld), int(nk))
+        if st != P3X_OK:
+            raise P3Error(st, "p3x_reserve")
 
     def last_kernel_times(self) -> tuple[float, float]:
         """Device microseconds of the last main() in two intervals: (the part 1 and part 2

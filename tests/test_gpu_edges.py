@@ -162,3 +162,19 @@ def test_bad_temperature_is_reported_not_fatal(ctx, oracle_tables, bad):
     got, _, _ = run_gpu(ctx, good, 300.0, True, False)
     errs = field_errors(got, want)
     assert max(errs.values()) <= RTOL_FP64, errs
+
+
+def test_reserve_then_smaller_sets(ctx, oracle_tables):
+    """p3x_reserve sizes the scratch once; later calls within it give exact results,
+    and shapes beyond the supported levels are refused."""
+    with pytest.raises(p3x.P3Error):
+        ctx.reserve(100, 513)
+    with pytest.raises(p3x.P3Error):
+        ctx.reserve(0, 72)
+    ctx.reserve(4096, 128)
+    for ncol, nlev, seed in ((300, 72, 101), (1000, 128, 102)):
+        inp = make_columns(ncol, nlev, seed=seed, dry_every=7, evap_every=9)
+        want, _, _ = oracle.run_oracle(inp, oracle_tables, 300.0, True, False, math=MATH)
+        got, _, _ = run_gpu(ctx, inp, 300.0, True, False)
+        errs = field_errors(got, want)
+        assert max(errs.values()) <= RTOL_FP64, errs
