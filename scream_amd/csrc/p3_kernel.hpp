@@ -76,6 +76,126 @@ constexpr int kMaxSubsteps = 100000;
 // Column status bits: part 1 found work, part 2 left hydrometeors.
 constexpr int kStatusPart1 = 1, kStatusPart2 = 2, kStatusFull = 3;
 
+// p3_main_init + p3_main_part1 (P3I/p3_main_impl_part1.hpp:18-185) for level k of
+// column i; returns its flags: bit 0 nucleationPossible, bit 1 hydrometeorsPresent.
+template <class R>
+__device__ __forceinline__ int part1_level(const KParams& p, int k, int i, unsigned& err) {
+  const int nlev = p.nlev;
+  const size_t ld = (size_t)p.ld;
+  const double dt = p.dt;
+  const double latvap = LatVap, latice = LatIce, inv_cp = INV_CP;
+  const Runtime& rt = p.rt;
+  auto scr = [&](int f, int kk) -> double& { return p.scratch[((size_t)f * nlev + kk) * ld + i]; };
+  const size_t x = (size_t)k * ld + i;
+  int flags = 0;
+  double inv_exner, th, qv, qc, nc, qr, nr, qi, ni, qm, bm, dz, cfl;
+  if (p.amble.on) {
+    // p3_preamble, eamxx_p3_process_interface.hpp:63-124
+    const Amble& a = p.amble;
+    const double pmid = a.p_mid[x], T = a.T_mid[x], cft = a.cldfrac_tot[x];
+    const double dp = a.dp_wet[x], dp_dry = p.dpres[x];
+    const double c1 = -1.0 + 1.0 / ep_2;  // calculate_dz with the wet qv's virtual temperature
+    const double T_virtual = T * (1.0 + c1 * p.qv[x]);
+    dz = (Rair / gravit) * dp * T_virtual / pmid;
+    a.dz[x] = dz;
+    // wet -> dry mixing ratios (calculate_drymmr_from_wetmmr_dp_based)
+    qc = p.qc[x] * dp / dp_dry; nc = p.nc[x] * dp / dp_dry; qr = p.qr[x] * dp / dp_dry;
+    nr = p.nr[x] * dp / dp_dry; qi = p.qi[x] * dp / dp_dry; ni = p.ni[x] * dp / dp_dry;
+    qm = p.qm[x] * dp / dp_dry; bm = p.bm[x] * dp / dp_dry;
+    qv = p.qv[x] * dp / dp_dry;
+    a.qv_prev[x] = a.qv_prev[x] * dp / dp_dry;
+    inv_exner = 1.0 / exner_function(pmid);
+    a.inv_exner[x] = inv_exner;
+    th = T / exner_function(pmid);
+    // cloud fractions with the 1e-4 floor and the maximum-overlap rain fraction
+    cfl = cft > 0.0001 ? cft : 0.0001;
+    a.cld_frac_l[x] = cfl;
+    a.cld_frac_i[x] = cfl;
+    double cfr = cfl;
+    if (k != 0) {
+      const double above = a.cldfrac_tot[x - ld];
+      cfr = above > cfr ? above : cfr;
+    }
+    a.rainfrac[x] = cfr;
+  } else {
+    inv_exner = p.inv_exner[x];
+    th = p.th[x];
+    qv = p.qv[x];
+    qc = p.qc[x]; nc = p.nc[x]; qr = p.qr[x]; nr = p.nr[x];
+    qi = p.qi[x]; ni = p.ni[x]; qm = p.qm[x]; bm = p.bm[x];
+    dz = p.dz[x];
+    cfl = p.prescribedCCN ? p.cld_frac_l[x] : 1.0;
+  }
+  // the state as it is in memory: a field part 1 leaves bit for bit unchanged
+  // is not stored again (with the pre-amble every field changes to dry)
+  const double in[10] = {qc, nc, qr, nr, qi, ni, qm, bm, qv, th};
+  qv = vmax(qv, 0.0);
+  const double exner = 1 / inv_exner;
+  const double pres = p.pres[x];
+  double T_atm = th * exner;
+  const double rho = p.dpres[x] / dz / gravit;
+  const double inv_rho = 1 / rho;
+  const double qv_sat_l = qv_sat_dry(T_atm, pres, false, err);
+  const double qv_sat_i = qv_sat_dry(T_atm, pres, true, err);
+  const double qv_supersat_i = qv / qv_sat_i - 1;
+  double rhofacr, rhofaci, acn;
+  {
+    const R ir = R(inv_rho), Tr = R(T_atm);
+    rhofacr = double(m_pow(R(RHO_1000MB) * ir, R(.54)));
+    rhofaci = double(m_pow(R(RHO_600MB) * ir, R(.54)));
+    const R dum = R(1.496e-6) * m_pow(Tr, R(1.5)) / (Tr + R(120));
+    acn = double(R(gravit) * R(RHO_H2O) / (R(18) * dum));
+  }
+  if (T_atm < T_zerodegc && qv_supersat_i >= -0.05) flags |= 1;
+
+  if (qc < QSMALL) {
+    qv = qv + qc;
+    th = th - inv_exner * qc * latvap * inv_cp;
+    qc = 0; nc = 0;
+  } else {
+    flags |= 2;
+    if (p.prescribedCCN) nc = vmax(nc, rt.spa_ccn_to_nc_factor * p.nccn[x] / (1 / cfl));
+    else if (p.predictNc) nc = vmax(nc + p.nc_nuceat_tend[x] * dt, 0.0);
+    else nc = NCCNST * inv_rho;
+  }
+  if (qr < QSMALL) {
+    qv = qv + qr;
+    th = th - inv_exner * qr * latvap * inv_cp;
+    qr = 0; nr = 0;
+  } else {
+    flags |= 2;
+  }
+  if (qi < QSMALL || (qi < 1.e-8 && qv_supersat_i < -0.1)) {
+    qv = qv + qi;
+    th = th - inv_exner * qi * (latvap + latice) * inv_cp;
+    qi = 0; ni = 0; qm = 0; bm = 0;
+  } else {
+    flags |= 2;
+  }
+  if (qi >= QSMALL && qi < 1.e-8 && T_atm >= T_zerodegc) {
+    qr = qr + qi;
+    th = th - inv_exner * qi * latice * inv_cp;
+    qi = 0; ni = 0; qm = 0; bm = 0;
+  }
+  T_atm = th * exner;
+
+  {
+    double* const f[10] = {p.qc, p.nc, p.qr, p.nr, p.qi, p.ni, p.qm, p.bm, p.qv, p.th};
+    const double out[10] = {qc, nc, qr, nr, qi, ni, qm, bm, qv, th};
+#pragma unroll
+    for (int q = 0; q < 10; ++q)
+      if (p.amble.on || __double_as_longlong(out[q]) != __double_as_longlong(in[q])) f[q][x] = out[q];
+  }
+  scr(SC_T, k) = T_atm;
+  scr(SC_QVSL, k) = qv_sat_l;
+  scr(SC_QVSI, k) = qv_sat_i;
+  scr(SC_SUPI, k) = qv_supersat_i;
+  scr(SC_RHOFACR, k) = rhofacr;
+  scr(SC_RHOFACI, k) = rhofaci;
+  scr(SC_ACN, k) = acn;
+  return flags;
+}
+
 // One block's work: columns bid*CB .. bid*CB+CB-1 of the column set p.
 // s_i is an LDS array of NT ints.
 template <class R, int CB, int NT>
@@ -92,14 +212,7 @@ __device__ __forceinline__ void part1_block(const KParams& p, int bid, int* s_i)
   const int L = (nlev + NCH - 1) / NCH;
   const int k0 = ch * L < nlev ? ch * L : nlev;
   const int k1 = k0 + L < nlev ? k0 + L : nlev;
-  const size_t ld = (size_t)p.ld;
-  const double dt = p.dt;
-  const double latvap = LatVap, latice = LatIce, inv_cp = INV_CP;
-  const Runtime& rt = p.rt;
   unsigned err = 0;
-
-  auto scr = [&](int f, int k) -> double& { return p.scratch[((size_t)f * nlev + k) * ld + i]; };
-  auto at = [&](int k) { return (size_t)k * ld + i; };
 
   auto red_or = [&](int v) {
     s_i[tid] = v;
@@ -110,117 +223,9 @@ __device__ __forceinline__ void part1_block(const KParams& p, int bid, int* s_i)
     return r;
   };
 
-  // ================= p3_main_init + p3_main_part1 (P3I/p3_main_impl_part1.hpp:18-185)
   int flags = 0;  // bit0 nucleationPossible, bit1 hydrometeorsPresent
   if (col_ok) {
-    for (int k = k0; k < k1; ++k) {
-      const size_t x = at(k);
-      double inv_exner, th, qv, qc, nc, qr, nr, qi, ni, qm, bm, dz, cfl;
-      if (p.amble.on) {
-        // p3_preamble, eamxx_p3_process_interface.hpp:63-124
-        const Amble& a = p.amble;
-        const double pmid = a.p_mid[x], T = a.T_mid[x], cft = a.cldfrac_tot[x];
-        const double dp = a.dp_wet[x], dp_dry = p.dpres[x];
-        const double c1 = -1.0 + 1.0 / ep_2;  // calculate_dz with the wet qv's virtual temperature
-        const double T_virtual = T * (1.0 + c1 * p.qv[x]);
-        dz = (Rair / gravit) * dp * T_virtual / pmid;
-        a.dz[x] = dz;
-        // wet -> dry mixing ratios (calculate_drymmr_from_wetmmr_dp_based)
-        qc = p.qc[x] * dp / dp_dry; nc = p.nc[x] * dp / dp_dry; qr = p.qr[x] * dp / dp_dry;
-        nr = p.nr[x] * dp / dp_dry; qi = p.qi[x] * dp / dp_dry; ni = p.ni[x] * dp / dp_dry;
-        qm = p.qm[x] * dp / dp_dry; bm = p.bm[x] * dp / dp_dry;
-        qv = p.qv[x] * dp / dp_dry;
-        a.qv_prev[x] = a.qv_prev[x] * dp / dp_dry;
-        inv_exner = 1.0 / exner_function(pmid);
-        a.inv_exner[x] = inv_exner;
-        th = T / exner_function(pmid);
-        // cloud fractions with the 1e-4 floor and the maximum-overlap rain fraction
-        cfl = cft > 0.0001 ? cft : 0.0001;
-        a.cld_frac_l[x] = cfl;
-        a.cld_frac_i[x] = cfl;
-        double cfr = cfl;
-        if (k != 0) {
-          const double above = a.cldfrac_tot[x - ld];
-          cfr = above > cfr ? above : cfr;
-        }
-        a.rainfrac[x] = cfr;
-      } else {
-        inv_exner = p.inv_exner[x];
-        th = p.th[x];
-        qv = p.qv[x];
-        qc = p.qc[x]; nc = p.nc[x]; qr = p.qr[x]; nr = p.nr[x];
-        qi = p.qi[x]; ni = p.ni[x]; qm = p.qm[x]; bm = p.bm[x];
-        dz = p.dz[x];
-        cfl = p.prescribedCCN ? p.cld_frac_l[x] : 1.0;
-      }
-      // the state as it is in memory: a field part 1 leaves bit for bit unchanged
-      // is not stored again (with the pre-amble every field changes to dry)
-      const double in[10] = {qc, nc, qr, nr, qi, ni, qm, bm, qv, th};
-      qv = vmax(qv, 0.0);
-      const double exner = 1 / inv_exner;
-      const double pres = p.pres[x];
-      double T_atm = th * exner;
-      const double rho = p.dpres[x] / dz / gravit;
-      const double inv_rho = 1 / rho;
-      const double qv_sat_l = qv_sat_dry(T_atm, pres, false, err);
-      const double qv_sat_i = qv_sat_dry(T_atm, pres, true, err);
-      const double qv_supersat_i = qv / qv_sat_i - 1;
-      double rhofacr, rhofaci, acn;
-      {
-        const R ir = R(inv_rho), Tr = R(T_atm);
-        rhofacr = double(m_pow(R(RHO_1000MB) * ir, R(.54)));
-        rhofaci = double(m_pow(R(RHO_600MB) * ir, R(.54)));
-        const R dum = R(1.496e-6) * m_pow(Tr, R(1.5)) / (Tr + R(120));
-        acn = double(R(gravit) * R(RHO_H2O) / (R(18) * dum));
-      }
-      if (T_atm < T_zerodegc && qv_supersat_i >= -0.05) flags |= 1;
-
-      if (qc < QSMALL) {
-        qv = qv + qc;
-        th = th - inv_exner * qc * latvap * inv_cp;
-        qc = 0; nc = 0;
-      } else {
-        flags |= 2;
-        if (p.prescribedCCN) nc = vmax(nc, rt.spa_ccn_to_nc_factor * p.nccn[x] / (1 / cfl));
-        else if (p.predictNc) nc = vmax(nc + p.nc_nuceat_tend[x] * dt, 0.0);
-        else nc = NCCNST * inv_rho;
-      }
-      if (qr < QSMALL) {
-        qv = qv + qr;
-        th = th - inv_exner * qr * latvap * inv_cp;
-        qr = 0; nr = 0;
-      } else {
-        flags |= 2;
-      }
-      if (qi < QSMALL || (qi < 1.e-8 && qv_supersat_i < -0.1)) {
-        qv = qv + qi;
-        th = th - inv_exner * qi * (latvap + latice) * inv_cp;
-        qi = 0; ni = 0; qm = 0; bm = 0;
-      } else {
-        flags |= 2;
-      }
-      if (qi >= QSMALL && qi < 1.e-8 && T_atm >= T_zerodegc) {
-        qr = qr + qi;
-        th = th - inv_exner * qi * latice * inv_cp;
-        qi = 0; ni = 0; qm = 0; bm = 0;
-      }
-      T_atm = th * exner;
-
-      {
-        double* const f[10] = {p.qc, p.nc, p.qr, p.nr, p.qi, p.ni, p.qm, p.bm, p.qv, p.th};
-        const double out[10] = {qc, nc, qr, nr, qi, ni, qm, bm, qv, th};
-#pragma unroll
-        for (int q = 0; q < 10; ++q)
-          if (p.amble.on || __double_as_longlong(out[q]) != __double_as_longlong(in[q])) f[q][x] = out[q];
-      }
-      scr(SC_T, k) = T_atm;
-      scr(SC_QVSL, k) = qv_sat_l;
-      scr(SC_QVSI, k) = qv_sat_i;
-      scr(SC_SUPI, k) = qv_supersat_i;
-      scr(SC_RHOFACR, k) = rhofacr;
-      scr(SC_RHOFACI, k) = rhofaci;
-      scr(SC_ACN, k) = acn;
-    }
+    for (int k = k0; k < k1; ++k) flags |= part1_level<R>(p, k, i, err);
   }
   const int cflags = red_or(flags);
   if (col_ok && ch == 0) p.colstat[i] = (cflags & 3) ? kStatusPart1 : 0;
@@ -237,6 +242,34 @@ __global__ __launch_bounds__(NT) void p3_part1_kernel(const KParams p) {
 constexpr int kPart2Cols = 64, kPart2Levels = 4;
 __host__ __device__ __forceinline__ int part2_tiles(int ncol, int nlev) {
   return ((ncol + kPart2Cols - 1) / kPart2Cols) * ((nlev + kPart2Levels - 1) / kPart2Levels);
+}
+
+// Part 1 with one thread per (level, column), on the part 2 tiles: a column's
+// status is the OR of its levels' flags, set atomically in a word that
+// p3_colstat_clear_kernel zeroed before.  Same per-level arithmetic as
+// part1_block; many more wavefronts, so the chip stays busy.
+template <class R>
+__device__ __forceinline__ void part1_tile(const KParams& p, int tile) {
+  const int ntc = (p.ncol + kPart2Cols - 1) / kPart2Cols;
+  const int i = (tile % ntc) * kPart2Cols + (threadIdx.x & 63);
+  const int k = (tile / ntc) * kPart2Levels + (threadIdx.x >> 6);
+  if (i >= p.ncol || k >= p.nlev) return;
+  unsigned err = 0;
+  if (part1_level<R>(p, k, i, err) & 3) atomicOr(&p.colstat[i], kStatusPart1);
+  if (err) atomicOr(p.err, err);
+}
+
+template <class R>
+__global__ __launch_bounds__(kPart2Cols * kPart2Levels) void p3_part1_lvl_kernel(const KParams p) {
+  part1_tile<R>(p, blockIdx.x);
+}
+
+// Zeroes the column status words before p3_part1_lvl_kernel (templated only so
+// that each precision's translation unit has its own copy).
+template <class R>
+__global__ __launch_bounds__(256) void p3_colstat_clear_kernel(int* colstat, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) colstat[i] = 0;
 }
 
 // ================= p3_main_part2 (P3I/p3_main_impl_part2.hpp:113-524) for the

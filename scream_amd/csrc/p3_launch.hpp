@@ -16,6 +16,7 @@ namespace p3d {
 //     holding L = ceil(nlev / lpc) levels (at most 8), 4 wavefronts per block.
 struct Shape {
   int cb, nt, lpc, wpe;
+  int p1lvl;  // 1: part 1 one thread per (level, column); 0: the cb x nt block kernel
 };
 
 // Largest levels-per-lane count of the sedimentation kernel.
