@@ -38,6 +38,30 @@ __global__ __launch_bounds__(NT) void p3_part1_batch(const KParams* __restrict__
   }
 }
 
+// Zeroes every set's column status words before p3_part1_lvl_batch: block b
+// takes sets b, b + gridDim.x, ...
+template <class R>
+__global__ __launch_bounds__(256) void p3_colstat_clear_batch(const KParams* __restrict__ sets, int nsets) {
+  const auto sets4 = (const __attribute__((address_space(4))) KParams*)sets;
+  for (int s = blockIdx.x; s < nsets; s += gridDim.x) {
+    const KParams& p = *(const KParams*)(sets4 + s);
+    for (int i = threadIdx.x; i < p.ncol; i += blockDim.x) p.colstat[i] = 0;
+  }
+}
+
+// Part 1 one thread per (level, column); first: the part 2 tile offsets.
+template <class R>
+__global__ __launch_bounds__(kPart2Cols * kPart2Levels) void p3_part1_lvl_batch(const KParams* __restrict__ sets,
+                                                                               const int* __restrict__ first,
+                                                                               int nsets) {
+  const auto sets4 = (const __attribute__((address_space(4))) KParams*)sets;
+  const int total = first[nsets];
+  for (int vb = blockIdx.x; vb < total; vb += gridDim.x) {
+    const int s = find_set(first, nsets, vb);
+    part1_tile<R>(*(const KParams*)(sets4 + s), vb - first[s]);
+  }
+}
+
 // first: the part 2 tile offsets (part2_tiles per set).
 template <class R, int WPE>
 __global__ __launch_bounds__(kPart2Cols * kPart2Levels, WPE) void p3_part2_batch(const KParams* __restrict__ sets,
