@@ -432,9 +432,11 @@ def summarize_pmc(fetch_dir, write_dir, out_path=None):
     scale_f = COPY_PROBE_BYTES / copy_f  # bytes per counter KiB, reads
     scale_w = COPY_PROBE_BYTES / copy_w  # bytes per counter KiB, writes
     kern = {}
-    for name in ("p3_part1_kernel", "p3_part2_kernel", "p3_sed_kernel"):
-        f = pick(fetch, name)
-        w = pick(write, name)
+    # part 1 is p3_part1_lvl_kernel (one thread per level and column) or p3_part1_kernel
+    for key, name in (("p3_part1", "p3_part1_kernel"), ("p3_part2", "p3_part2_kernel"),
+                      ("p3_sed", "p3_sed_kernel")):
+        f = pick(fetch, key)
+        w = pick(write, key)
         if f and w:
             kern[name] = {"FETCH_SIZE_KiB": max(f.values()), "WRITE_SIZE_KiB": max(w.values()),
                           "bytes": max(f.values()) * scale_f + max(w.values()) * scale_w}

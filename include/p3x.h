@@ -280,7 +280,7 @@ int p3x_process_create(p3x_context* ctx, const p3x_runtime* runtime, int ncol, i
                        int predictNc, int prescribedCCN, p3x_process** process);
 /* run_impl(dt): pre-amble, p3_main, post-amble (same stream, same call); the
  * pre-amble runs inside p3_main's first kernel and the post-amble inside its
- * last, so a step is three kernel launches. */
+ * last, so a step launches the same kernels as p3x_main. */
 int p3x_process_run(p3x_process* process, const p3x_atm_fields* fields, double dt,
                     const p3x_launch* opts);
 void p3x_process_destroy(p3x_process* process);

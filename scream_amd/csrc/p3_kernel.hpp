@@ -1,12 +1,15 @@
 // p3_kernel.hpp -- p3_main_init, part 1 and part 2 of a P3 step (the first
 // two of its three kernels; sedimentation and part 3 are in p3_sed.hpp).
 //
-// Part 1 (p3_part1_kernel): a workgroup owns CB consecutive columns; its NT =
-// 64*waves threads split every column's levels into NCH = NT/CB contiguous
-// chunks; thread t handles column c = t % CB (so CB consecutive lanes read CB
-// consecutive doubles of a level-major row) and chunk t / CB.  The column's
-// early-exit flag (nucleationPossible || hydrometeorsPresent) is an
-// OR-reduction through LDS; the column status goes to KParams::colstat.
+// Part 1 (part1_level) runs in one of two kernels:
+//  * p3_part1_lvl_kernel, one thread per (level, column) on the part 2 tiles
+//    below; the column's early-exit flag (nucleationPossible ||
+//    hydrometeorsPresent) is the OR of its levels' flags, set with atomicOr
+//    into status words p3_colstat_clear_kernel zeroed first;
+//  * p3_part1_kernel, a workgroup of CB consecutive columns whose NT threads
+//    split every column's levels into NCH = NT/CB contiguous chunks (thread t:
+//    column t % CB, chunk t / CB), the flag an OR-reduction through LDS.
+// Either way the column status goes to KParams::colstat.
 //
 // Part 2 (p3_part2_kernel): pointwise in the level once the column's part-1
 // status is known, so one thread per (level, column): a workgroup is 64

@@ -4,7 +4,7 @@
 // temperature change, dry -> wet, effective radii in microns, the accumulated
 // surface precipitation masses and the optional boundary fluxes).  The pre-amble
 // runs inside the part 1 kernel and the post-amble inside the sedimentation +
-// part 3 kernel (p3d::Amble), so a process step is the three p3_main kernels.
+// part 3 kernel (p3d::Amble), so a process step launches the p3_main kernels only.
 //
 // Follows components/eamxx/src/physics/p3/eamxx_p3_process_interface.hpp:63-261,
 // eamxx_p3_process_interface.cpp:249-340 (which fields feed p3_main) and
