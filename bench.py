@@ -116,7 +116,7 @@ def time_steps(ctx, w, dt, precision, steps, warmup):
 
 def kernel_split(ctx, w, dt, precision, steps):
     """Mean device microseconds of a step in two intervals, (the part 1 and part 2 kernels, the
-    sedimentation + part 3 kernel), from HIP events the library records on the kernels'
+    sedimentation and part 3 kernels), from HIP events the library records on the kernels'
     stream around and between them."""
     import torch
     sp = torch.cuda.current_stream().cuda_stream
@@ -434,7 +434,7 @@ def summarize_pmc(fetch_dir, write_dir, out_path=None):
     kern = {}
     # part 1 is p3_part1_lvl_kernel (one thread per level and column) or p3_part1_kernel
     for key, name in (("p3_part1", "p3_part1_kernel"), ("p3_part2", "p3_part2_kernel"),
-                      ("p3_sed", "p3_sed_kernel")):
+                      ("p3_sed", "p3_sed_kernel"), ("p3_part3", "p3_part3_kernel")):
         f = pick(fetch, key)
         w = pick(write, key)
         if f and w:
@@ -496,7 +496,9 @@ def main():
     colev = ncol * nlev
     value = colev / (ms / 1000.0)
     achieved = SED_BYTES_PER_COLUMN_LEVEL * colev / (sed_us * 1e-6) / 1e9
-    traffic = traffic_for("g64_fp64", "p3_sed_kernel")
+    # the dominant stage: sedimentation, then part 3 in its own kernel
+    t_sed, t_p3 = traffic_for("g64_fp64", "p3_sed_kernel"), traffic_for("g64_fp64", "p3_part3_kernel")
+    traffic = t_sed + t_p3 if (t_sed is not None and t_p3 is not None) else None
     workload = f"p3_main FP64 drop-in, {ncol}x{nlev}, dt={dt:g} s"
     line = {
         "metric": METRIC, "value": value, "unit": "column-levels/s", "n_gpus": 1,
@@ -509,11 +511,11 @@ def main():
                    "copy_GBps": copy_gbs,
                    "step_frac_of_copy": BYTES_PER_COLUMN_LEVEL * colev / (kern_us * 1e-6) / 1e9 / copy_gbs,
                    "step_traffic_bytes": traffic_for("g64_fp64")},
-        # the dominant kernel (sedimentation + part 3): algorithmic bytes per launch over its
-        # event-timed duration, and its measured HBM bytes per launch
+        # the dominant stage (the sedimentation kernel and the part 3 kernel after it):
+        # algorithmic bytes per step over its event-timed duration, and its measured HBM bytes
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "p3_sed_kernel"},
+                     "kernel": "p3_sed_kernel + p3_part3_lvl_kernel"},
     }
     del w
     if not args.no_process:

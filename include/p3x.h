@@ -200,7 +200,7 @@ int p3x_main(p3x_context* ctx, const p3x_runtime* runtime,
 int p3x_sync(p3x_context* ctx);
 
 /* Device time of the context's last p3x_main (waits for it) in two intervals:
- * the part 1 and part 2 kernels, then the sedimentation + part 3 kernel.  No reference counterpart (the
+ * the part 1 and part 2 kernels, then the sedimentation and part 3 kernels.  No reference counterpart (the
  * reference times the whole p3_main, p3_main_impl.hpp:328-330); it splits that
  * figure for profiling. */
 int p3x_last_kernel_times(p3x_context* ctx, double* part12_us, double* sed_us);

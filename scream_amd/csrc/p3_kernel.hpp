@@ -68,6 +68,7 @@ struct KParams {
   double dt;
   int predictNc, prescribedCCN, exch_zero;
   double e_melt;        // saturation vapour pressure over liquid at Tmelt (qv_sat_dry(Tmelt, p) numerator)
+  int part3_in_sed;     // 1: the sedimentation kernel runs part 3; 0: p3_part3_lvl_kernel does
   Amble amble;
 };
 

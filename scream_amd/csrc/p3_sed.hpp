@@ -550,13 +550,26 @@ __device__ __forceinline__ void sed_block(const KParams& p, int bid) {
       }
     }
   }
-  // the sedimented fields, stored above by other lanes of the block, are read below
-  __syncthreads();
-  for (int e = threadIdx.x; e < CB * nlev; e += 64 * WPB) {
-    const int k = e / CB, c = e % CB, ic = col0 + c;
-    if (ic < p.ncol) part3_level<R>(p, k, ic, p.colstat[ic] == kStatusFull);
+  if (p.part3_in_sed) {
+    // the sedimented fields, stored above by other lanes of the block, are read below
+    __syncthreads();
+    for (int e = threadIdx.x; e < CB * nlev; e += 64 * WPB) {
+      const int k = e / CB, c = e % CB, ic = col0 + c;
+      if (ic < p.ncol) part3_level<R>(p, k, ic, p.colstat[ic] == kStatusFull);
+    }
   }
   if (err) atomicOr(p.err, err);
+}
+
+// Part 3 as a kernel of its own, one thread per (level, column) on the part 2
+// tiles, after the sedimentation kernel (part3_in_sed = 0).
+template <class R>
+__global__ __launch_bounds__(kPart2Cols * kPart2Levels) void p3_part3_lvl_kernel(const KParams p) {
+  const int ntc = (p.ncol + kPart2Cols - 1) / kPart2Cols;
+  const int i = ((int)blockIdx.x % ntc) * kPart2Cols + (threadIdx.x & 63);
+  const int k = ((int)blockIdx.x / ntc) * kPart2Levels + (threadIdx.x >> 6);
+  if (i >= p.ncol || k >= p.nlev) return;
+  part3_level<R>(p, k, i, p.colstat[i] == kStatusFull);
 }
 
 // Up to 5 levels per lane the kernel fits two wavefronts per SIMD (256 registers).

@@ -134,6 +134,7 @@ void fill_params(const p3x_context* c, const p3x_runtime* runtime, const p3x_pro
   unsigned no_err = 0;
   k.e_melt = p3d::mk_svp(p3c::Tmelt, false, no_err);
   k.amble = p3d::Amble{};  // no fused pre/post-amble
+  k.part3_in_sed = 1;
 }
 
 // Every field pointer of a column set is non-null.
@@ -321,6 +322,11 @@ int p3x_detail::main_impl(p3x_context* c, const p3x_runtime* runtime, const p3x_
   fill_params(c, runtime, ps, di, dout, infra, hist, nj, nk, ld, o.col_status, o.sed_substeps,
               o.ice_table_idx, c->d_scratch, &k);
   if (amble) k.amble = *amble;
+  // part 3 in a kernel of its own, one thread per (level, column): at 16384 x 128
+  // the sedimentation + part 3 time fell 9 % (FP64) and 17 % (mixed), at
+  // 21600 x 72 by 1-4 %.  The batched and served forms keep it in the
+  // sedimentation kernel.
+  k.part3_in_sed = 0;
 
   // Every call on this context shares its scratch: a call on another stream than
   // the previous one waits on the device for the previous one to end.

@@ -363,7 +363,7 @@ class Context:
 
     def last_kernel_times(self) -> tuple[float, float]:
         """Device microseconds of the last main() in two intervals: (the part 1 and part 2
-        kernels, the sedimentation + part 3 kernel)."""
+        kernels, the sedimentation and part 3 kernels)."""
         a, b = C.c_double(), C.c_double()
         st = self._lib.p3x_last_kernel_times(self.handle, C.byref(a), C.byref(b))
         if st != P3X_OK:
