@@ -75,6 +75,25 @@ __global__ __launch_bounds__(kPart2Cols * kPart2Levels, WPE) void p3_part2_batch
   }
 }
 
+// Part 3 one thread per (level, column), after the sedimentation launches;
+// first: the part 2 tile offsets.
+template <class R>
+__global__ __launch_bounds__(kPart2Cols * kPart2Levels) void p3_part3_lvl_batch(const KParams* __restrict__ sets,
+                                                                               const int* __restrict__ first,
+                                                                               int nsets) {
+  const auto sets4 = (const __attribute__((address_space(4))) KParams*)sets;
+  const int total = first[nsets];
+  for (int vb = blockIdx.x; vb < total; vb += gridDim.x) {
+    const int s = find_set(first, nsets, vb);
+    const KParams& p = *(const KParams*)(sets4 + s);
+    const int tile = vb - first[s];
+    const int ntc = (p.ncol + kPart2Cols - 1) / kPart2Cols;
+    const int i = (tile % ntc) * kPart2Cols + (threadIdx.x & 63);
+    const int k = (tile / ntc) * kPart2Levels + (threadIdx.x >> 6);
+    if (i < p.ncol && k < p.nlev) part3_level<R>(p, k, i, p.colstat[i] == kStatusFull);
+  }
+}
+
 template <class R, int LPC, int L, int WPB>
 __global__ __launch_bounds__(64 * WPB, sed_waves_per_simd<L>()) void p3_sed_batch(const KParams* __restrict__ sets,
                                                          const int* __restrict__ first, int nsets) {

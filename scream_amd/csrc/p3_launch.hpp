@@ -56,6 +56,7 @@ struct BatchPlan {
   const int* first2;    // device array, nsets + 1 entries: part 2 tiles
   int nsets;
   int grid, grid2;      // persistent blocks of the part 1 and part 2 launches
+  int part3_sep;        // 1: part 3 in p3_part3_lvl_batch on the part 2 tiles (the sets say part3_in_sed = 0)
   int nclass;
   SedClass cls[kMaxLevelsPerLane];
 };

@@ -502,6 +502,7 @@ int build_plan(p3x_batch* b) {
   b->plan.first = b->d_first;
   b->plan.first2 = b->d_first + (n + 1);
   b->plan.nsets = n;
+  b->plan.part3_sep = 1;
   b->plan.nclass = (int)lay.cls.size();
   for (int ci = 0; ci < b->plan.nclass; ++ci) {
     const Layout::Cls& lc = lay.cls[ci];
@@ -592,6 +593,7 @@ extern "C" int p3x_batch_create(p3x_context* c, const p3x_runtime* runtime, cons
       fill_params(c, runtime, &s.prognostic_state, &s.diagnostic_inputs, &s.diagnostic_outputs,
                   &s.infrastructure, &s.history_only, s.nj, s.nk, ld, s.col_status, s.sed_substeps,
                   s.ice_table_idx, (double*)((char*)b->d_scratch + offs[i]), &b->sets[i]);
+      b->sets[i].part3_in_sed = 0;  // part 3 in its own launch after sedimentation
     }
     e = hipMalloc((void**)&b->d_sets, nsets * sizeof(p3d::KParams));
   }

@@ -134,6 +134,7 @@ int launch_slot(p3x_server* srv, Slot& sl) {
   plan.first = (const int*)(sl.d_desc + o_first);
   plan.first2 = (const int*)(sl.d_desc + o_first2);
   plan.nsets = n;
+  plan.part3_sep = 1;
   plan.grid = grid_for(srv->ctx->ncu, 0, lay.first[n], srv->occ1);
   plan.grid2 = grid_for(srv->ctx->ncu, 0, lay.first2[n], srv->occ2);
   plan.nclass = (int)lay.cls.size();
@@ -350,6 +351,7 @@ extern "C" int p3x_server_add_member(p3x_server* s, const p3x_column_set* set, i
   fill_params(s->ctx, &s->rt, &set->prognostic_state, &set->diagnostic_inputs, &set->diagnostic_outputs, &inf,
               &set->history_only, set->nj, set->nk, ld, set->col_status, set->sed_substeps, set->ice_table_idx,
               nullptr, &m.k);
+  m.k.part3_in_sed = 0;  // part 3 in its own launch after sedimentation
   m.scratch = need;
   m.busy = false;
   s->members.push_back(m);
