@@ -1,5 +1,5 @@
 // p3_kernel.hpp -- p3_main_init, part 1 and part 2 of a P3 step (the first
-// two of its three kernels; sedimentation and part 3 are in p3_sed.hpp).
+// two of its four kernels; sedimentation and part 3 are in p3_sed.hpp).
 //
 // Part 1 (part1_level) runs in one of two kernels:
 //  * p3_part1_lvl_kernel, one thread per (level, column) on the part 2 tiles

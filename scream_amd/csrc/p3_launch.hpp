@@ -7,13 +7,16 @@
 
 namespace p3d {
 
-// Shape of one step's three launches.
-//   part 1: cb columns per block x nt threads per block (compiled: 16x128,
+// Shape of one step's launches.
+//   part 1: p1lvl = 1, one thread per (level, column) on the part 2 tiles;
+//     else cb columns per block x nt threads per block (compiled: 16x128,
 //     16x256, 32x256, 64x256, 32x512, 64x512);
 //   part 2: 64 columns x 4 levels per block, register allocation for wpe = 1
 //     or 2 wavefronts per SIMD (2: fewer registers, with spills in FP64);
-//   sedimentation + part 3: lpc lanes per column (16, 32 or 64), each lane
-//     holding L = ceil(nlev / lpc) levels (at most 8), 4 wavefronts per block.
+//   sedimentation: lpc lanes per column (16, 32 or 64), each lane
+//     holding L = ceil(nlev / lpc) levels (at most 8), 4 wavefronts per block;
+//   part 3: one thread per (level, column) on the part 2 tiles (or inside the
+//     sedimentation kernel when KParams::part3_in_sed is set).
 struct Shape {
   int cb, nt, lpc, wpe;
   int p1lvl;  // 1: part 1 one thread per (level, column); 0: the cb x nt block kernel
