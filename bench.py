@@ -59,16 +59,20 @@ def parse():
 
 
 class Workload:
-    """Device-resident fields for one column set, plus a pristine copy of the state."""
+    """Device-resident fields for one column set, plus a pristine copy of the state.
+    layout: level-major (nlev, ncol) arrays, or column-major (ncol, nlev), the order of
+    the reference's field-manager views."""
 
-    def __init__(self, ncol, nlev, seed):
+    def __init__(self, ncol, nlev, seed, layout=0):
         import numpy as np
         import torch
         from scream_amd import p3x
         from scream_amd.inputs import make_columns
-        self.ncol, self.nlev = ncol, nlev
+        self.ncol, self.nlev, self.layout = ncol, nlev, layout
         dev = torch.device("cuda:0")
         inp = make_columns(ncol, nlev, seed=seed)
+        if layout == p3x.P3X_COLUMN_MAJOR:
+            inp = {k: np.ascontiguousarray(v.T) for k, v in inp.items()}
         # the 10 state fields are views of one allocation, so that restoring the
         # pristine state before a step is a single device copy
         ns = len(p3x.STATE)
@@ -76,7 +80,8 @@ class Workload:
         self.f = {k: self.state[j] for j, k in enumerate(p3x.STATE)}
         self.f.update({k: torch.from_numpy(inp[k]).to(dev) for k in p3x.INPUTS})
         for k in p3x.OUTS + p3x.HIST:
-            shape = (ncol,) if "surf" in k else ((nlev + 1, ncol) if "flux" in k else (nlev, ncol))
+            rows = nlev + 1 if "flux" in k else nlev
+            shape = (ncol,) if "surf" in k else ((ncol, rows) if layout else (rows, ncol))
             self.f[k] = torch.zeros(shape, dtype=torch.float64, device=dev)
         self.pristine = self.state.clone()
         assert self.state.shape[0] == ns
@@ -93,7 +98,7 @@ def time_steps(ctx, w, dt, precision, steps, warmup):
     for _ in range(warmup):
         w.restore()
         ctx.main(w.f, w.ncol, w.nlev, dt, True, False, precision=precision, stream=sp,
-                 async_=True, exchange_zeroed=True)
+                 async_=True, exchange_zeroed=True, layout=w.layout)
     torch.cuda.synchronize()
     evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
            for _ in range(steps)]
@@ -103,7 +108,7 @@ def time_steps(ctx, w, dt, precision, steps, warmup):
         w.restore()
         e0.record(stream)
         ctx.main(w.f, w.ncol, w.nlev, dt, True, False, precision=precision, stream=sp,
-                 async_=True, exchange_zeroed=True)
+                 async_=True, exchange_zeroed=True, layout=w.layout)
         e1.record(stream)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
@@ -124,18 +129,19 @@ def kernel_split(ctx, w, dt, precision, steps):
     for _ in range(steps):
         w.restore()
         ctx.main(w.f, w.ncol, w.nlev, dt, True, False, precision=precision, stream=sp,
-                 async_=True, exchange_zeroed=True)
+                 async_=True, exchange_zeroed=True, layout=w.layout)
         x, y = ctx.last_kernel_times()
         a.append(x)
         b.append(y)
     return statistics.mean(a), statistics.mean(b)
 
 
-def process_step(ctx, ncol, nlev, dt, steps, warmup, seed=1):
+def process_step(ctx, ncol, nlev, dt, steps, warmup, seed=1, layout=0):
     """The P3Microphysics process step (run_impl: pre-amble, p3_main, post-amble, the
     pre/post-amble fused into the p3_main kernels) on wet field-manager fields of the
     same generator: wall ms/step and HIP-event kernel us/step, the updated fields
-    restored from one pristine copy before every step (outside the events)."""
+    restored from one pristine copy before every step (outside the events).  layout:
+    level-major, or column-major (the field manager's (column, level) views, dense)."""
     import numpy as np
     import torch
     from scream_amd import p3x
@@ -143,6 +149,8 @@ def process_step(ctx, ncol, nlev, dt, steps, warmup, seed=1):
                                    make_atm_fields)
     dev = torch.device("cuda:0")
     f = make_atm_fields(ncol, nlev, seed=seed)
+    if layout == p3x.P3X_COLUMN_MAJOR:
+        f = {k: np.ascontiguousarray(v.T) for k, v in f.items()}
     flat = torch.from_numpy(np.concatenate([f[k].ravel() for k in ATM_UPDATED_FIELDS])).to(dev)
     d, off = {}, 0
     for k in ATM_UPDATED_FIELDS:
@@ -151,11 +159,11 @@ def process_step(ctx, ncol, nlev, dt, steps, warmup, seed=1):
         off += n
     d.update({k: torch.from_numpy(v).to(dev) for k, v in f.items() if k not in d})
     for k in ATM_COMPUTED_FIELDS:
-        d[k] = torch.zeros((nlev, ncol), dtype=torch.float64, device=dev)
+        d[k] = torch.zeros((ncol, nlev) if layout else (nlev, ncol), dtype=torch.float64, device=dev)
     for k in ATM_FLUX_FIELDS:
         d[k] = torch.zeros((ncol,), dtype=torch.float64, device=dev)
     pristine = flat.clone()
-    proc = p3x.P3Microphysics(ctx, ncol, nlev, predict_nc=True, prescribed_ccn=False)
+    proc = p3x.P3Microphysics(ctx, ncol, nlev, predict_nc=True, prescribed_ccn=False, layout=layout)
     stream = torch.cuda.current_stream()
     sp = stream.cuda_stream
     for _ in range(warmup):
@@ -178,7 +186,9 @@ def process_step(ctx, ncol, nlev, dt, steps, warmup, seed=1):
         raise RuntimeError(f"p3x status {st}")
     ms = 1000.0 * (t1 - t0) / steps
     kern_us = statistics.mean(1000.0 * e0.elapsed_time(e1) for e0, e1 in evs)
-    return {"workload": f"P3Microphysics run_impl FP64, {ncol}x{nlev}, dt={dt:g} s, boundary fluxes on",
+    lay = "column-major" if layout == p3x.P3X_COLUMN_MAJOR else "level-major"
+    return {"workload": f"P3Microphysics run_impl FP64, {ncol}x{nlev}, dt={dt:g} s, boundary fluxes on, "
+                        f"{lay} fields",
             "column_levels_per_s": ncol * nlev / (ms / 1000.0), "ms_per_step": ms,
             "kernel_us_per_step": kern_us}
 
@@ -205,54 +215,116 @@ def mixed_error(ctx, w, dt):
     return out
 
 
-def batched(ctx, nsets, steps, seed=4):
-    """Batched form: nsets independent column sets, ncol log-uniform in [256, 8192],
-    nlev from {72, 128}, dt from {100, 300, 1800} s, mixed precision, one graph replay
-    per step; launch shape from the autotuner.  The sets' state advances step after
-    step (no restore), as in a run."""
-    import numpy as np
+class SetBatch:
+    """Device-resident column sets for the batched form: every set's 10 state fields are
+    views of one allocation, so that restoring all of them is one device copy."""
+
+    def __init__(self, specs, seed0, inputs=None):
+        import numpy as np
+        import torch
+        from scream_amd import p3x
+        from scream_amd.inputs import make_columns
+        dev = torch.device("cuda:0")
+        self.specs = specs  # (ncol, nlev, dt)
+        inps = inputs or [make_columns(n, k, seed=seed0 + s) for s, (n, k, _) in enumerate(specs)]
+        total = sum(10 * n * k for n, k, _ in specs)
+        self.state = torch.empty(total, dtype=torch.float64, device=dev)
+        self.fields, off = [], 0
+        for inp, (n, k, _) in zip(inps, specs):
+            f = {}
+            for key in p3x.STATE:
+                f[key] = self.state[off:off + n * k].view(k, n)
+                f[key].copy_(torch.from_numpy(inp[key]))
+                off += n * k
+            f.update({key: torch.from_numpy(inp[key]).to(dev) for key in p3x.INPUTS})
+            for key in p3x.OUTS + p3x.HIST:
+                shape = (n,) if "surf" in key else ((k + 1, n) if "flux" in key else (k, n))
+                f[key] = torch.zeros(shape, dtype=torch.float64, device=dev)
+            self.fields.append(f)
+        self.pristine = self.state.clone()
+        self.column_levels = float(sum(n * k for n, k, _ in specs))
+
+    def restore(self):
+        self.state.copy_(self.pristine)
+
+    def column_sets(self):
+        from scream_amd import p3x
+        return [p3x.column_set(f, n, k, dt, exchange_zeroed=True) for f, (n, k, dt) in zip(self.fields, self.specs)]
+
+
+def run_batched(ctx, sb, precision, steps, tune_reps=2):
+    """One graph-captured batch of the sets of ``sb``, launch shape from the autotuner; the
+    state of every set is restored before each step (inside the wall clock, outside the
+    events around the batch).  Beside it, the same sets stepped one p3x_main call after
+    another (``singles``, same restore)."""
     import torch
     from scream_amd import p3x
-    from scream_amd.inputs import make_columns
-    rng = np.random.default_rng(seed)
-    ncols = np.round(np.exp(rng.uniform(np.log(256), np.log(8192), nsets))).astype(int)
-    nlevs = rng.choice([72, 128], nsets)
-    dts = rng.choice([100.0, 300.0, 1800.0], nsets)
-    dev = torch.device("cuda:0")
-    keep, sets = [], []
-    for s in range(nsets):
-        n, k = int(ncols[s]), int(nlevs[s])
-        inp = make_columns(n, k, seed=2000 + s)
-        f = {key: torch.from_numpy(inp[key]).to(dev) for key in p3x.STATE + p3x.INPUTS}
-        for key in p3x.OUTS + p3x.HIST:
-            shape = (n,) if "surf" in key else ((k + 1, n) if "flux" in key else (k, n))
-            f[key] = torch.zeros(shape, dtype=torch.float64, device=dev)
-        keep.append(f)
-        sets.append(p3x.column_set(f, n, k, float(dts[s]), exchange_zeroed=True))
-    torch.cuda.synchronize()
-    b = p3x.Batch(ctx, sets, p3x.batch_defaults(precision=p3x.P3X_MIXED, use_graph=1))
     sp = torch.cuda.current_stream().cuda_stream
-    chosen, tuned_us = b.tune(sp, reps=2)
+    b = p3x.Batch(ctx, sb.column_sets(), p3x.batch_defaults(precision=precision, use_graph=1))
+    chosen, tuned_us = b.tune(sp, reps=tune_reps)
     for _ in range(2):
+        sb.restore()
         b.run(sp)
     torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
     t0 = time.perf_counter()
-    e0.record()
-    for _ in range(steps):
+    for e0, e1 in evs:
+        sb.restore()
+        e0.record()
         b.run(sp)
-    e1.record()
+        e1.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     st = ctx.sync()
     b.close()
     if st != 0:
         raise RuntimeError(f"batched form: p3x status {st}")
-    colev = float((ncols * nlevs).sum())
-    return {"sets": nsets, "column_levels_per_step": colev, "steps": steps,
-            "ms_per_step": 1000.0 * wall / steps, "kernel_ms_per_step": e0.elapsed_time(e1) / steps,
-            "column_levels_per_s": colev * steps / wall, "precision": "mixed", "graph": True,
-            "tuned": chosen.as_dict(), "tuned_us_per_step": tuned_us}
+    kern_ms = statistics.mean(e0.elapsed_time(e1) for e0, e1 in evs)
+    # the same sets through p3x_main, one call per set
+    def singles():
+        for f, (n, k, dt) in zip(sb.fields, sb.specs):
+            ctx.main(f, n, k, dt, True, False, precision=precision, stream=sp, async_=True,
+                     exchange_zeroed=True)
+    sb.restore()
+    singles()
+    torch.cuda.synchronize()
+    sev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    for e0, e1 in sev:
+        sb.restore()
+        e0.record()
+        singles()
+        e1.record()
+    torch.cuda.synchronize()
+    if ctx.sync() != 0:
+        raise RuntimeError("batched form: singles failed")
+    singles_ms = statistics.mean(e0.elapsed_time(e1) for e0, e1 in sev)
+    return {"sets": len(sb.specs), "column_levels_per_step": sb.column_levels, "steps": steps,
+            "ms_per_step": 1000.0 * wall / steps, "kernel_ms_per_step": kern_ms,
+            "column_levels_per_s": sb.column_levels / (kern_ms / 1000.0),
+            "wall_column_levels_per_s": sb.column_levels * steps / wall,
+            "singles_ms_per_step": singles_ms, "batched_over_singles": singles_ms / kern_ms,
+            "precision": "mixed" if precision == p3x.P3X_MIXED else "fp64", "graph": True,
+            "tuned": chosen.as_dict(), "tuned_us_per_step": tuned_us,
+            "state": "restored before every step (one device copy, outside the events)"}
+
+
+def batched(ctx, nsets, steps, seed=4):
+    """Batched form: nsets independent column sets, ncol log-uniform in [256, 8192],
+    nlev from {72, 128}, dt from {100, 300, 1800} s, mixed precision, one graph replay
+    per step; launch shape from the autotuner."""
+    import numpy as np
+    from scream_amd import p3x
+    rng = np.random.default_rng(seed)
+    ncols = np.round(np.exp(rng.uniform(np.log(256), np.log(8192), nsets))).astype(int)
+    nlevs = rng.choice([72, 128], nsets)
+    dts = rng.choice([100.0, 300.0, 1800.0], nsets)
+    specs = [(int(n), int(k), float(dt)) for n, k, dt in zip(ncols, nlevs, dts)]
+    sb = SetBatch(specs, 2000)
+    out = run_batched(ctx, sb, p3x.P3X_MIXED, steps)
+    out["workload"] = (f"{nsets} column sets, ncol log-uniform 256-8192, nlev 72/128, dt 100/300/1800 s "
+                       f"(seed {seed})")
+    del sb
+    return out
 
 
 def served(ctx, seconds, rate=200.0, members=32, nlev=128, dt=300.0, seed=5):
@@ -303,12 +375,35 @@ def served(ctx, seconds, rate=200.0, members=32, nlev=128, dt=300.0, seed=5):
             "precision": "mixed", "slots": 4, "max_batch": 32}
 
 
+# One GPU's share of the GPU box's host CPUs: worker pools on the box are sized to it,
+# whatever the host's own CPU count is.
+CPU_SHARE = 16
+
+
+def host_facts():
+    """The host's CPU count, the CPUs this process may run on, and the CPU model."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return {"host_cpus": os.cpu_count(), "affinity_cpus": aff, "cpu_model": model, "cpu_share": CPU_SHARE}
+
+
 def cpu_baseline():
-    """The CPU path timed on this host on the headline workload itself (21600 columns x
-    72 levels, dt 1800 s, same generator), median of 5 runs after a discarded cold run.  With oracle/_ref built (the reference's own Fortran P3,
-    micro_p3.F90, compiled from its sources) that is the baseline ("reference"),
-    called on 64-column chunks from a thread pool; the C restatement of the oracle
-    (OpenMP over columns) is timed beside it ("port_value")."""
+    """The CPU path timed on this host, in the same run, on (1) the headline workload itself
+    (21600 columns x 72 levels, dt 1800 s, same generator) and (2) the reference's CPU
+    configuration (256 x 72, dt 300 s): the wall time of the step alone (no init, no table
+    load), median of repeats after a discarded cold run.  With oracle/_ref built (the
+    reference's own Fortran P3, micro_p3.F90, compiled from its sources) that is the
+    baseline ("reference"), called on column chunks from a pool of threads, one per CPU of
+    this GPU's share of the host; the C restatement of the oracle (OpenMP over columns, same
+    threads) is timed beside it ("port_value")."""
     import threading
     from concurrent.futures import ThreadPoolExecutor
 
@@ -322,36 +417,47 @@ def cpu_baseline():
     tabs = dict(ice=a(t.ice_table, (5, 4, 50, 12)), coll=a(t.collect_table, (5, 4, 50, 30, 2)),
                 vn=a(t.vn_table, (300, 10)), vm=a(t.vm_table, (300, 10)),
                 revap=a(t.revap_table, (300, 10)), mu_r=a(t.mu_r_table, (150,)), dnu=a(t.dnu, (16,)))
-    ncol, nlev, dt = 21600, 72, 1800.0
-    inp = make_columns(ncol, nlev, seed=1)
-    nthreads = max(1, min(16, os.cpu_count() or 1))
+    facts = host_facts()
+    nthreads = max(1, min(CPU_SHARE, facts["affinity_cpus"]))
 
-    def median_time(fn):
+    def median_time(fn, reps):
         fn()
         times = []
-        for _ in range(5):
+        for _ in range(reps):
             t0 = time.perf_counter()
             fn()
             times.append(time.perf_counter() - t0)
         return statistics.median(times)
 
-    port = median_time(lambda: oracle.run_oracle(inp, tabs, dt, True, False, nthreads=nthreads))
-    out = {"value": ncol * nlev / port, "unit": "column-levels/s", "cores": nthreads, "kind": "port",
-           "sample": f"oracle/p3_oracle.c FP64 (OpenMP), {ncol}x{nlev} columns of the same generator, "
-                     f"dt 1800 s, median of 5 runs ({port:.3f} s each)"}
-    if oracle.ref_available():
-        table_dir = os.path.join(ROOT, "scream_amd", "data")
-        chunks = [{k: np.ascontiguousarray(v[:, c:c + 64]) for k, v in inp.items()} for c in range(0, ncol, 64)]
+    have_ref = oracle.ref_available()
+    table_dir = os.path.join(ROOT, "scream_amd", "data")
+    if have_ref:
         oracle.load_ref(table_dir)  # p3_init (table file I/O) once, before any thread calls p3_main
         threading.stack_size(256 << 20)  # the Fortran keeps its column arrays on the stack
-        with ThreadPoolExecutor(nthreads) as ex:
-            run = lambda: list(ex.map(lambda ch: oracle.run_ref(ch, table_dir, dt, True, False), chunks))
-            ref = median_time(run)
-        out = {"value": ncol * nlev / ref, "unit": "column-levels/s", "cores": nthreads, "kind": "reference",
-               "sample": f"reference Fortran p3_main (micro_p3.F90 via oracle/_ref), {ncol}x{nlev} columns of "
-                         f"the same generator in 64-column chunks on {nthreads} threads, dt 1800 s, median of "
-                         f"5 runs ({ref:.3f} s each)",
-               "port_value": out["value"], "port_sample": out["sample"]}
+
+    def leg(ncol, nlev, dt, seed, chunk, reps):
+        inp = make_columns(ncol, nlev, seed=seed)
+        port = median_time(lambda: oracle.run_oracle(inp, tabs, dt, True, False, nthreads=nthreads), reps)
+        r = {"workload": f"{ncol}x{nlev}, dt {dt:g} s", "port_value": ncol * nlev / port, "port_s": port}
+        if have_ref:
+            chunks = [{k: np.ascontiguousarray(v[:, c:c + chunk]) for k, v in inp.items()}
+                      for c in range(0, ncol, chunk)]
+            with ThreadPoolExecutor(nthreads) as ex:
+                run = lambda: list(ex.map(lambda ch: oracle.run_ref(ch, table_dir, dt, True, False), chunks))
+                ref = median_time(run, reps)
+            r.update({"value": ncol * nlev / ref, "ref_s": ref, "chunk_columns": chunk})
+        return r
+
+    g64 = leg(21600, 72, 1800.0, 1, 64, 5)
+    cpu_cfg = leg(256, 72, 300.0, 3, 16, 51)
+    kind = "reference" if have_ref else "port"
+    what = ("reference Fortran p3_main (micro_p3.F90 via oracle/_ref)" if have_ref
+            else "oracle/p3_oracle.c FP64 (OpenMP)")
+    out = {"value": g64["value"] if have_ref else g64["port_value"], "unit": "column-levels/s",
+           "cores": nthreads, "kind": kind,
+           "sample": f"{what}, the headline workload 21600x72 dt 1800 s (same generator), whole, on "
+                     f"{nthreads} threads (one GPU's share of the host), median of 5 runs after a cold run",
+           "port_value": g64["port_value"], "g64": g64, "cpu_config": cpu_cfg, **facts}
     return out
 
 
@@ -480,6 +586,7 @@ def main():
               "P3 columns are independent, so N GPUs would each run this workload on their "
               "own column range", file=sys.stderr)
         sys.exit(2)
+    import numpy as np
     import torch
     from scream_amd import p3x
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
@@ -518,8 +625,18 @@ def main():
                      "kernel": "p3_sed_kernel + p3_part3_lvl_kernel"},
     }
     del w
+    # the same step on the reference's own (column, level) layout of the fields
+    wc = Workload(ncol, nlev, seed=1, layout=p3x.P3X_COLUMN_MAJOR)
+    ms_c, kern_c = time_steps(ctx, wc, dt, p3x.P3X_FP64, args.steps, args.warmup)
+    line["column_major"] = {"workload": workload + ", column-major (column, level) arrays",
+                            "column_levels_per_s": colev / (ms_c / 1000.0), "ms_per_step": ms_c,
+                            "kernel_us_per_step": kern_c, "kernel_time_over_level_major": kern_c / kern_us}
+    del wc
     if not args.no_process:
         line["process"] = process_step(ctx, ncol, nlev, dt, args.steps, args.warmup)
+        pc = process_step(ctx, ncol, nlev, dt, args.steps, args.warmup, layout=p3x.P3X_COLUMN_MAJOR)
+        pc["kernel_time_over_level_major"] = pc["kernel_us_per_step"] / line["process"]["kernel_us_per_step"]
+        line["process_column_major"] = pc
     if not args.no_forms:
         ncol2, nlev2, dt2 = 16384, 128, 100.0
         w2 = Workload(ncol2, nlev2, seed=2)
@@ -530,6 +647,20 @@ def main():
                            "kernel_us_per_step": k2,
                            "hbm_frac": BYTES_PER_COLUMN_LEVEL * ncol2 * nlev2 / (k2 * 1e-6) / 8e12}
         forms["mixed_over_fp64"] = forms["mixed"]["column_levels_per_s"] / forms["fp64"]["column_levels_per_s"]
+        forms["mixed_over_fp64_kernel"] = forms["fp64"]["kernel_us_per_step"] / forms["mixed"]["kernel_us_per_step"]
+        # the mixed form batched: the same 16384 x 128 columns as 8 sets of 2048 (one graph
+        # replay per step, autotuned), against the plain FP64 drop-in on the whole set
+        from scream_amd.inputs import make_columns
+        whole = make_columns(ncol2, nlev2, seed=2)
+        parts = [{k: np.ascontiguousarray(v[..., j * (ncol2 // 8):(j + 1) * (ncol2 // 8)])
+                  for k, v in whole.items()} for j in range(8)]
+        sb = SetBatch([(ncol2 // 8, nlev2, dt2)] * 8, 0, inputs=parts)
+        del whole, parts
+        mb = run_batched(ctx, sb, p3x.P3X_MIXED, max(5, args.steps // 2))
+        del sb
+        mb["workload"] = f"mixed batched form, 8 column sets of {ncol2 // 8} x {nlev2}, dt={dt2:g} s"
+        mb["over_fp64_drop_in"] = forms["fp64"]["kernel_us_per_step"] / (1000.0 * mb["kernel_ms_per_step"])
+        forms["mixed_batched"] = mb
         errs = mixed_error(ctx, w2, dt2)
         forms["mixed_error_vs_fp64"] = {
             "worst_field": max(errs, key=lambda k: errs[k]["max_rel_to_field_max"]),

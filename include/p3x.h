@@ -14,12 +14,19 @@
  *   init_kokkos_ice_lookup_tables / init_kokkos_tables  impl/p3_table_ice_impl.hpp:16-78,
  *                                                       impl/p3_table3_impl.hpp:85-148
  *
- * Data layout.  Every per-level field is a DEVICE pointer to a level-major
- * array: element (column i, level k) lives at [k*ld + i], k = 0 at the model
- * top, 0 <= i < nj, ld >= nj.  The two precipitation-flux fields have nk+1
- * rows (interfaces).  precip_liq_surf / precip_ice_surf hold nj values.
- * (The reference's Kokkos views are (column, pack) LayoutRight; the level-major
- * layout is what lets consecutive GPU lanes read consecutive columns.)
+ * Data layout.  Every per-level field is a DEVICE pointer, k = 0 at the model
+ * top, 0 <= i < nj, in one of two layouts (p3x_launch.layout):
+ *   P3X_COLUMN_MAJOR  the reference's Kokkos views (column, pack) LayoutRight
+ *                     (eamxx_p3_process_interface.cpp:254-265): element
+ *                     (column i, level k) at [i*ld + k], ld >= nk; the two
+ *                     precipitation-flux fields have nk+1 levels (interfaces)
+ *                     at [i*ld_iface + k], ld_iface >= nk+1.  A pack-padded
+ *                     view passes its padded extents as ld and ld_iface.
+ *   P3X_LEVEL_MAJOR   (the default) element (i, k) at [k*ld + i], ld >= nj,
+ *                     flux rows k = 0..nk at the same stride.
+ * precip_liq_surf / precip_ice_surf hold nj values.  Both layouts give the
+ * same results bit for bit; the kernels map their lanes so that loads
+ * coalesce in either.
  *
  * Field names, units and meanings are those of the reference structs.
  * Semantics follow the reference's p3_main exactly, with one documented
@@ -51,7 +58,10 @@ enum {
   P3X_ERR_TEMPERATURE = 5,  /* T <= 0 or NaN reached the saturation functions;     */
                             /* the reference aborts here (physics_saturation_impl.hpp:14-36) */
   P3X_ERR_NO_MEMORY = 6,    /* device allocation failed                            */
-  P3X_ERR_SUBSTEPS = 7      /* a sedimentation loop hit the 100000-substep cap     */
+  P3X_ERR_SUBSTEPS = 7,     /* a sedimentation loop hit the 100000-substep cap     */
+  P3X_ERR_BUDGET = 8,       /* served form: a member does not fit the HBM budget   */
+  P3X_ERR_CHECK_VALUES = 9  /* with p3x_launch.check_values: a level's T left     */
+                            /* (173, 323) K after the step (p3_check_values_impl.hpp:25-79) */
 };
 
 /* ---- P3Runtime (p3_functions.hpp:112-158); defaults from p3x_runtime_defaults ---- */
@@ -137,9 +147,11 @@ typedef struct p3x_table_data {
 
 /* ---- launch options (no reference counterpart: how the step runs on MI355X) ---- */
 enum { P3X_FP64 = 0, P3X_MIXED = 1 };
+enum { P3X_LEVEL_MAJOR = 0, P3X_COLUMN_MAJOR = 1 };
 typedef struct p3x_launch {
   int precision;           /* P3X_FP64 (drop-in) or P3X_MIXED (FP32 bulk rates)  */
-  int ld;                  /* row stride of the level-major arrays; 0 means nj    */
+  int ld;                  /* level-major: row stride (>= nj; 0 means nj);        */
+                           /* column-major: column stride (>= nk; 0 means nk)     */
   /* part 1 kernel: columns per block x threads per block, both 0 for the
    * default, else one of 16x128, 16x256, 32x256, 64x256, 32x512, 64x512      */
   int columns_per_block;
@@ -155,10 +167,21 @@ typedef struct p3x_launch {
   /* optional device arrays for the integer diagnostics (NULL = off):
    *   col_status [nj]        0 skipped after part 1, 1 after part 2, 2 full
    *   sed_substeps [3*nj]    CFL substeps of cloud, rain, ice sedimentation
-   *   ice_table_idx [nk*ld]  part-3 ice-table cell jj*10000+ii*100+i or -1      */
+   *   ice_table_idx          part-3 ice-table cell jj*10000+ii*100+i or -1, a
+   *                          level field in the layout of the others             */
   int* col_status;
   int* sed_substeps;
   int* ice_table_idx;
+  int layout;              /* P3X_LEVEL_MAJOR (0) or P3X_COLUMN_MAJOR              */
+  int ld_iface;            /* column-major: column stride of the two nk+1-level   */
+                           /* flux fields (0 means max(ld, nk+1)); level-major: 0 */
+  /* optional device array [nj] (NULL = off): the reference's debug-build
+   * check_values after part 3 (p3_main_impl.hpp:316-324, bounds at
+   * p3_check_values_impl.hpp:25-79) on the fully stepped columns: bit 0 = some
+   * level's T = th exner is outside (173, 323) K (the trap: the step then
+   * returns P3X_ERR_CHECK_VALUES), bit 1 = some level's qv is outside
+   * (0, 40e-3) (a flag only, as in the reference)                             */
+  int* check_values;
 } p3x_launch;
 
 typedef struct p3x_context p3x_context;
@@ -217,8 +240,10 @@ typedef struct p3x_column_set {
   p3x_diagnostic_outputs diagnostic_outputs;
   p3x_history_only history_only;
   p3x_infrastructure infrastructure;  /* dt, predictNc, prescribedCCN, exchange_zeroed_on_entry */
-  int nj, nk, ld;                     /* ld 0 means nj                                       */
+  int nj, nk, ld;                     /* ld as in p3x_launch (0 = dense)                     */
   int *col_status, *sed_substeps, *ice_table_idx;  /* optional, as in p3x_launch         */
+  int layout, ld_iface;               /* as in p3x_launch                                    */
+  int* check_values;                  /* optional, as in p3x_launch                          */
 } p3x_column_set;
 
 typedef struct p3x_batch_options {
@@ -253,8 +278,9 @@ void p3x_batch_destroy(p3x_batch* batch);
 /* ---- the P3Microphysics atmosphere process (eamxx_p3_process_interface.hpp:22-444,
  * eamxx_p3_run.cpp:5-45): pre-amble, p3_main, post-amble on the field-manager
  * fields.  Names are the field names the process requests
- * (eamxx_p3_process_interface.cpp:24-121); every array is a level-major device
- * array as above, per-column fields have ncol entries. ---- */
+ * (eamxx_p3_process_interface.cpp:24-121); every per-level array is a device
+ * array in the layout the process was created with, per-column fields have ncol
+ * entries. ---- */
 typedef struct p3x_atm_fields {
   /* required inputs */
   const double *p_mid, *p_dry_mid, *pseudo_density, *pseudo_density_dry, *cldfrac_tot;
@@ -275,24 +301,35 @@ typedef struct p3x_process p3x_process;
 
 /* set_grids + initialize_impl: one process for ncol x nlev columns; allocates
  * the process-local arrays (inv_exner, th_atm, cld_frac_l/i, dz, the
- * precipitation fluxes, qv2qi_depos_tend, rho_qi). */
+ * precipitation fluxes, qv2qi_depos_tend, rho_qi) in the fields' layout.
+ * p3x_process_create_layout takes the layout of the field-manager fields:
+ * P3X_COLUMN_MAJOR with the views' column strides (ld for the nlev-level
+ * fields, ld_iface for nlev+1-level ones; 0 = dense), as get_view<Pack**>()
+ * hands them to the reference (eamxx_p3_process_interface.cpp:254-265);
+ * p3x_process_create is the level-major, dense form. */
 int p3x_process_create(p3x_context* ctx, const p3x_runtime* runtime, int ncol, int nlev,
                        int predictNc, int prescribedCCN, p3x_process** process);
+int p3x_process_create_layout(p3x_context* ctx, const p3x_runtime* runtime, int ncol, int nlev,
+                              int predictNc, int prescribedCCN, int layout, int ld, int ld_iface,
+                              p3x_process** process);
 /* run_impl(dt): pre-amble, p3_main, post-amble (same stream, same call); the
  * pre-amble runs inside p3_main's first kernel and the post-amble inside its
- * last, so a step launches the same kernels as p3x_main. */
+ * last, so a step launches the same kernels as p3x_main.  The layout is the
+ * process's own: opts->layout, ld and ld_iface must be 0 or equal to it. */
 int p3x_process_run(p3x_process* process, const p3x_atm_fields* fields, double dt,
                     const p3x_launch* opts);
 void p3x_process_destroy(p3x_process* process);
 
 /* ---- the mass-and-energy column conservation check that runs after the process
  * (share/property_checks/mass_and_energy_column_conservation_check.cpp:110-353).
- * Device arrays, level-major with ld = ncol; levels are summed k = 0 .. nlev-1.
- * Both calls are enqueued on stream (NULL = default) and return at once. ---- */
+ * Per-level device arrays in the given layout and stride (as p3x_launch; ld 0 =
+ * dense); levels are summed k = 0 .. nlev-1.  Both calls are enqueued on
+ * stream (NULL = default) and return at once. ---- */
 typedef struct p3x_budget_fields {
   const double *pseudo_density, *T_mid, *u, *v;  /* u, v: horiz_winds; NULL = 0     */
   const double *qv, *qc, *qi, *qr;
   const double *ps, *phis;                       /* ncol each; NULL = 0             */
+  int layout, ld;                                /* P3X_LEVEL_MAJOR (0) or column-major */
 } p3x_budget_fields;
 
 /* Column total mass [kg/m2] and total energy [J/m2] (:283-301, :312-340). */
@@ -311,21 +348,25 @@ int p3x_conservation_errors(p3x_context* ctx, const double* mass_before, const d
  * cld_fraction_main_impl.hpp:12-95): ice cloud fraction 1 where qi exceeds a
  * threshold (strictly) and 0 elsewhere, total = max(ice, liquid), for the
  * model's threshold (default 1e-12) and the analysis threshold (default 1e-5).
- * Level-major device arrays with ld = ncol; enqueued on stream (NULL =
- * default), returns at once. ---- */
+ * Pointwise over ncol*nlev dense device arrays, so either layout (all six
+ * arrays in the same one); enqueued on stream (NULL = default), returns at
+ * once. ---- */
 int p3x_cld_fraction(p3x_context* ctx, const double* qi, const double* cldfrac_liq, int ncol, int nlev,
                      double ice_threshold, double ice_4out_threshold, double* cldfrac_ice,
                      double* cldfrac_tot, double* cldfrac_ice_4out, double* cldfrac_tot_4out,
                      void* stream);
 
 /* ---- served form (no reference counterpart): step requests from concurrent
- * ensemble members, continuously batched.  Members register once; their
- * arrays (caller-owned device memory) stay resident and are advanced in place.
- * A scheduler thread admits queued requests into batches as launch slots and
- * scratch (a fixed HBM budget) free up, one request per member in flight at a
- * time, in arrival order otherwise; batches on different slots run
- * concurrently on their own HIP streams.  A request's result equals that of a
- * p3x_main call on its member alone. ---- */
+ * ensemble members, continuously batched.  Members register at any time, also
+ * while others are being served; their arrays (caller-owned device memory) stay
+ * resident and are advanced in place.  A member is admitted only while the
+ * bytes all members' arrays span plus the slots' scratch fit the HBM budget.  A
+ * scheduler thread admits queued requests into batches as launch slots and
+ * scratch free up, one request per member in flight at a time, in arrival
+ * order otherwise; batches on different slots run concurrently on their own
+ * HIP streams.  Each request has its own status (a step that fails -- T <= 0
+ * or NaN -- fails alone).  A request's result equals that of a p3x_main call
+ * on its member alone. ---- */
 typedef struct p3x_server_options {
   int precision;              /* P3X_FP64 or P3X_MIXED                                */
   int slots;                  /* batches in flight at once (one stream each); 0 = 4   */
@@ -333,6 +374,7 @@ typedef struct p3x_server_options {
   double scratch_budget_bytes;/* device scratch of all slots; 0 = 8 GiB               */
   int columns_per_block, threads_per_block, sed_lanes_per_column;  /* 0 = default  */
   int part12_waves_per_simd;                                        /* 0 = default  */
+  double hbm_budget_bytes;    /* members' resident arrays + scratch; 0 = 200e9       */
 } p3x_server_options;
 
 typedef struct p3x_completion {
@@ -347,6 +389,9 @@ typedef struct p3x_server_stats {
   double p50_us, p99_us, max_us;  /* over all completed requests                      */
   double column_levels;           /* total column-levels stepped                      */
   double busy_s;                  /* first submit -> last completion                  */
+  double resident_bytes;          /* bytes the registered members' arrays span        */
+  double scratch_bytes;           /* the slots' scratch                               */
+  int members;
 } p3x_server_stats;
 
 typedef struct p3x_server p3x_server;
@@ -354,8 +399,10 @@ typedef struct p3x_server p3x_server;
 void p3x_server_defaults(p3x_server_options* opts);
 int p3x_server_create(p3x_context* ctx, const p3x_runtime* runtime, const p3x_server_options* opts,
                       p3x_server** server);
-/* Register a member (its column set; infrastructure.dt is replaced per request).
- * P3X_ERR_ARG if its scratch can never fit one slot's share of the budget. */
+/* Register a member (its column set; infrastructure.dt is replaced per request);
+ * allowed while serving.  P3X_ERR_BUDGET if its arrays do not fit the HBM budget
+ * beside the members already registered, or its scratch can never fit one
+ * slot's share of the scratch budget. */
 int p3x_server_add_member(p3x_server* server, const p3x_column_set* set, int* member);
 /* Queue one step of member with time step dt; thread-safe. */
 int p3x_server_submit(p3x_server* server, int member, double dt, unsigned long long tag);

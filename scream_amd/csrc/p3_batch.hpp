@@ -49,9 +49,9 @@ __global__ __launch_bounds__(256) void p3_colstat_clear_batch(const KParams* __r
   }
 }
 
-// Part 1 one thread per (level, column); first: the part 2 tile offsets.
+// Part 1 one thread per (level, column); first: the pointwise tile offsets.
 template <class R>
-__global__ __launch_bounds__(kPart2Cols * kPart2Levels) void p3_part1_lvl_batch(const KParams* __restrict__ sets,
+__global__ __launch_bounds__(kTileThreads) void p3_part1_lvl_batch(const KParams* __restrict__ sets,
                                                                                const int* __restrict__ first,
                                                                                int nsets) {
   const auto sets4 = (const __attribute__((address_space(4))) KParams*)sets;
@@ -62,7 +62,7 @@ __global__ __launch_bounds__(kPart2Cols * kPart2Levels) void p3_part1_lvl_batch(
   }
 }
 
-// first: the part 2 tile offsets (part2_tiles per set).
+// first: the pointwise tile offsets (pw_tiles per set).
 template <class R, int WPE>
 __global__ __launch_bounds__(kPart2Cols * kPart2Levels, WPE) void p3_part2_batch(const KParams* __restrict__ sets,
                                                                                 const int* __restrict__ first,
@@ -76,21 +76,15 @@ __global__ __launch_bounds__(kPart2Cols * kPart2Levels, WPE) void p3_part2_batch
 }
 
 // Part 3 one thread per (level, column), after the sedimentation launches;
-// first: the part 2 tile offsets.
+// first: the pointwise tile offsets.
 template <class R>
-__global__ __launch_bounds__(kPart2Cols * kPart2Levels) void p3_part3_lvl_batch(const KParams* __restrict__ sets,
-                                                                               const int* __restrict__ first,
-                                                                               int nsets) {
+__global__ __launch_bounds__(kTileThreads) void p3_part3_lvl_batch(const KParams* __restrict__ sets,
+                                                                  const int* __restrict__ first, int nsets) {
   const auto sets4 = (const __attribute__((address_space(4))) KParams*)sets;
   const int total = first[nsets];
   for (int vb = blockIdx.x; vb < total; vb += gridDim.x) {
     const int s = find_set(first, nsets, vb);
-    const KParams& p = *(const KParams*)(sets4 + s);
-    const int tile = vb - first[s];
-    const int ntc = (p.ncol + kPart2Cols - 1) / kPart2Cols;
-    const int i = (tile % ntc) * kPart2Cols + (threadIdx.x & 63);
-    const int k = (tile / ntc) * kPart2Levels + (threadIdx.x >> 6);
-    if (i < p.ncol && k < p.nlev) part3_level<R>(p, k, i, p.colstat[i] == kStatusFull);
+    part3_tile<R>(*(const KParams*)(sets4 + s), vb - first[s]);
   }
 }
 

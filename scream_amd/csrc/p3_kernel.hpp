@@ -28,8 +28,9 @@
 
 namespace p3d {
 
-// Level-major scratch fields (nlev x ld doubles each) that carry per-level
-// results of part 1 to part 2 and to the sedimentation kernel.
+// Scratch fields (ncol x nlev doubles each, in the layout of the column set)
+// that carry per-level results of part 1 to part 2 and to the sedimentation
+// kernel.
 enum Scr { SC_T = 0, SC_QVSL, SC_QVSI, SC_SUPI, SC_RHOFACR, SC_RHOFACI, SC_ACN, SC_COUNT };
 
 // The P3Microphysics pre- and post-amble (eamxx_p3_process_interface.hpp:63-261),
@@ -58,18 +59,35 @@ struct KParams {
       *precip_total_tend, *nevapr;
   // P3HistoryOnly (in/out unless exch_zero)
   double *liq_ice_exchange, *vap_liq_exchange, *vap_ice_exchange;
-  double* scratch;      // SC_COUNT x nlev x ld
+  double* scratch;      // SC_COUNT fields of sfs doubles
   int* colstat;         // per column: 0 skipped after part 1, 1 after part 2, 3 full step
   unsigned* err;        // device error word
   int *dbg_status, *dbg_substeps, *dbg_ice_idx;
+  int* dbg_check;       // per column: check_values flags (NULL = the check is off)
   Tables tab;
   Runtime rt;
-  int ncol, nlev, ld;   // ld: row stride (elements) of every level-major array
+  int ncol, nlev;
+  // Layout.  Element (column i, level k) of a level field is at i*cs + k*ks, of
+  // an interface field (nlev + 1 levels) at i*fcs + k*fks: level-major
+  // (cs = fcs = 1, ks = fks = the row stride) or column-major, the reference's
+  // (column, level) LayoutRight views (ks = fks = 1, cs and fcs the column
+  // strides).  Scratch field f holds (i, k) at f*sfs + i*scs + k*sks, dense, in
+  // the same order.
+  int cs, ks, fcs, fks;
+  int scs, sks;
+  size_t sfs;
+  int colmajor;         // 1: column-major; the pointwise kernels then map lanes to levels
+  int dbg_stride;       // species stride of dbg_substeps (>= ncol)
   double dt;
   int predictNc, prescribedCCN, exch_zero;
   double e_melt;        // saturation vapour pressure over liquid at Tmelt (qv_sat_dry(Tmelt, p) numerator)
-  int part3_in_sed;     // 1: the sedimentation kernel runs part 3; 0: p3_part3_lvl_kernel does
   Amble amble;
+
+  __host__ __device__ __forceinline__ size_t at(int i, int k) const { return (size_t)i * cs + (size_t)k * ks; }
+  __host__ __device__ __forceinline__ size_t atf(int i, int k) const { return (size_t)i * fcs + (size_t)k * fks; }
+  __host__ __device__ __forceinline__ double& sc(int f, int i, int k) const {
+    return scratch[(size_t)f * sfs + (size_t)i * scs + (size_t)k * sks];
+  }
 };
 
 __host__ __device__ __forceinline__ double exner_function(double pres) {  // PhysicsFunctions, p^(Rd/cp)
@@ -84,13 +102,11 @@ constexpr int kStatusPart1 = 1, kStatusPart2 = 2, kStatusFull = 3;
 // column i; returns its flags: bit 0 nucleationPossible, bit 1 hydrometeorsPresent.
 template <class R>
 __device__ __forceinline__ int part1_level(const KParams& p, int k, int i, unsigned& err) {
-  const int nlev = p.nlev;
-  const size_t ld = (size_t)p.ld;
   const double dt = p.dt;
   const double latvap = LatVap, latice = LatIce, inv_cp = INV_CP;
   const Runtime& rt = p.rt;
-  auto scr = [&](int f, int kk) -> double& { return p.scratch[((size_t)f * nlev + kk) * ld + i]; };
-  const size_t x = (size_t)k * ld + i;
+  auto scr = [&](int f, int kk) -> double& { return p.sc(f, i, kk); };
+  const size_t x = p.at(i, k);
   int flags = 0;
   double inv_exner, th, qv, qc, nc, qr, nr, qi, ni, qm, bm, dz, cfl;
   if (p.amble.on) {
@@ -117,7 +133,7 @@ __device__ __forceinline__ int part1_level(const KParams& p, int k, int i, unsig
     a.cld_frac_i[x] = cfl;
     double cfr = cfl;
     if (k != 0) {
-      const double above = a.cldfrac_tot[x - ld];
+      const double above = a.cldfrac_tot[x - p.ks];
       cfr = above > cfr ? above : cfr;
     }
     a.rainfrac[x] = cfr;
@@ -233,6 +249,7 @@ __device__ __forceinline__ void part1_block(const KParams& p, int bid, int* s_i)
   }
   const int cflags = red_or(flags);
   if (col_ok && ch == 0) p.colstat[i] = (cflags & 3) ? kStatusPart1 : 0;
+  if (col_ok && ch == 0 && p.dbg_check) p.dbg_check[i] = 0;
   if (err) atomicOr(p.err, err);
 }
 
@@ -242,24 +259,65 @@ __global__ __launch_bounds__(NT) void p3_part1_kernel(const KParams p) {
   part1_block<R, CB, NT>(p, blockIdx.x, s_i);
 }
 
-// Part 2 tiles of a column set: 64 columns x 4 levels each, columns fastest.
-constexpr int kPart2Cols = 64, kPart2Levels = 4;
+// Tiles of the pointwise kernels (part 1, part 2, part 3): 256 threads, one
+// (level, column) point each.  Level-major: 64 consecutive columns x 4 levels,
+// one level per wavefront, columns fastest, so a wavefront reads 64
+// consecutive doubles of a row.  Column-major: 256 consecutive points of the
+// column-by-column order (levels fastest), so a wavefront reads consecutive
+// levels of one or two columns.
+constexpr int kPart2Cols = 64, kPart2Levels = 4, kTileThreads = kPart2Cols * kPart2Levels;
 __host__ __device__ __forceinline__ int part2_tiles(int ncol, int nlev) {
   return ((ncol + kPart2Cols - 1) / kPart2Cols) * ((nlev + kPart2Levels - 1) / kPart2Levels);
 }
+__host__ __device__ __forceinline__ int pw_tiles(const KParams& p) {
+  if (p.colmajor) return (int)(((long long)p.ncol * p.nlev + kTileThreads - 1) / kTileThreads);
+  return part2_tiles(p.ncol, p.nlev);
+}
+// The point of this thread in tile `tile`; false past the end of the set.
+// (Column-major point counts are below 2^31: the host refuses larger sets.)
+__device__ __forceinline__ bool pw_point(const KParams& p, int tile, int& i, int& k) {
+  if (p.colmajor) {
+    const unsigned e = (unsigned)tile * kTileThreads + threadIdx.x;
+    const unsigned nl = (unsigned)p.nlev;
+    i = (int)(e / nl);
+    k = (int)(e - (unsigned)i * nl);
+    return i < p.ncol;
+  }
+  const int ntc = (p.ncol + kPart2Cols - 1) / kPart2Cols;
+  i = (tile % ntc) * kPart2Cols + (threadIdx.x & 63);
+  k = (tile / ntc) * kPart2Levels + (threadIdx.x >> 6);
+  return i < p.ncol && k < p.nlev;
+}
 
-// Part 1 with one thread per (level, column), on the part 2 tiles: a column's
-// status is the OR of its levels' flags, set atomically in a word that
+// ORs `bit` into the status word of column i for every lane whose `set` is
+// true.  Level-major lanes hold distinct columns; column-major lanes hold runs
+// of consecutive levels of a column (lane l's run starts min(l, k) lanes
+// below it), so one lane per run and wavefront makes the atomic.  Every lane
+// of the wavefront that is still running must call this.
+__device__ __forceinline__ void colstat_or(const KParams& p, int i, int k, bool set, int bit) {
+  if (!p.colmajor) {
+    if (set) atomicOr(&p.colstat[i], bit);
+    return;
+  }
+  const unsigned long long m = __ballot(set);
+  const int l = threadIdx.x & 63;
+  const int below = k < l ? k : l;
+  const unsigned long long run_below = ((1ull << l) - 1) ^ ((1ull << (l - below)) - 1);
+  if (set && !(m & run_below)) atomicOr(&p.colstat[i], bit);
+}
+
+// Part 1 with one thread per (level, column), on the pointwise tiles: a
+// column's status is the OR of its levels' flags, set atomically in a word that
 // p3_colstat_clear_kernel zeroed before.  Same per-level arithmetic as
 // part1_block; many more wavefronts, so the chip stays busy.
 template <class R>
 __device__ __forceinline__ void part1_tile(const KParams& p, int tile) {
-  const int ntc = (p.ncol + kPart2Cols - 1) / kPart2Cols;
-  const int i = (tile % ntc) * kPart2Cols + (threadIdx.x & 63);
-  const int k = (tile / ntc) * kPart2Levels + (threadIdx.x >> 6);
-  if (i >= p.ncol || k >= p.nlev) return;
+  int i, k;
+  if (!pw_point(p, tile, i, k)) return;
   unsigned err = 0;
-  if (part1_level<R>(p, k, i, err) & 3) atomicOr(&p.colstat[i], kStatusPart1);
+  const int flags = part1_level<R>(p, k, i, err);
+  colstat_or(p, i, k, (flags & 3) != 0, kStatusPart1);
+  if (k == 0 && p.dbg_check) p.dbg_check[i] = 0;  // part 3 ORs the column's check_values flags
   if (err) atomicOr(p.err, err);
 }
 
@@ -280,16 +338,12 @@ __global__ __launch_bounds__(256) void p3_colstat_clear_kernel(int* colstat, int
 // level of this wavefront and the column of this lane in tile `tile`.
 template <class R>
 __device__ __forceinline__ void part2_tile(const KParams& p, int tile) {
-  const int ntc = (p.ncol + kPart2Cols - 1) / kPart2Cols;
-  const int i = (tile % ntc) * kPart2Cols + (threadIdx.x & 63);
-  const int k = (tile / ntc) * kPart2Levels + (threadIdx.x >> 6);
-  if (i >= p.ncol || k >= p.nlev) return;
-  const int nlev = p.nlev;
-  const size_t ld = (size_t)p.ld;
-  const size_t x = (size_t)k * ld + i;
-  auto scr = [&](int f) -> double { return p.scratch[((size_t)f * nlev + k) * ld + i]; };
+  int i, k;
+  if (!pw_point(p, tile, i, k)) return;
+  const size_t x = p.at(i, k);
+  auto scr = [&](int f) -> double { return p.sc(f, i, k); };
   unsigned err = 0;
-  bool done = false;
+  bool done = false, hydro = false;
   // The column status and the level's skip-test inputs are loaded together (one
   // memory round trip for the decision); bit 1 only: other levels of the column
   // may already have set bit 2.
@@ -318,8 +372,7 @@ __device__ __forceinline__ void part2_tile(const KParams& p, int tile) {
       s.ni_activated = p.ni_activated[x];
       const double in[10] = {s.qc, s.nc, s.qr, s.nr, s.qi, s.ni, s.qm, s.bm, s.qv, s.th};
       // hydrometeorsPresent after part 2 (P3I/p3_main_impl.hpp:255-260): a column flag
-      if (part2_level<R>(p.rt, p.tab, p.dt, p.predictNc, p.prescribedCCN, p.e_melt, s, err))
-        atomicOr(&p.colstat[i], kStatusPart2);
+      hydro = part2_level<R>(p.rt, p.tab, p.dt, p.predictNc, p.prescribedCCN, p.e_melt, s, err);
       {
         // mixed form: a state field whose bits part 2 left unchanged is not stored
         // again (in FP64 the ten held originals cost more registers than the
@@ -349,6 +402,8 @@ __device__ __forceinline__ void part2_tile(const KParams& p, int tile) {
       p.liq_ice_exchange[x] = 0;
     }
   }
+  // hydrometeorsPresent after part 2 (P3I/p3_main_impl.hpp:255-260): a column flag
+  colstat_or(p, i, k, hydro, kStatusPart2);
   if (err) atomicOr(p.err, err);
 }
 

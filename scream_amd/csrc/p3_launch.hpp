@@ -8,15 +8,15 @@
 namespace p3d {
 
 // Shape of one step's launches.
-//   part 1: p1lvl = 1, one thread per (level, column) on the part 2 tiles;
-//     else cb columns per block x nt threads per block (compiled: 16x128,
-//     16x256, 32x256, 64x256, 32x512, 64x512);
-//   part 2: 64 columns x 4 levels per block, register allocation for wpe = 1
-//     or 2 wavefronts per SIMD (2: fewer registers, with spills in FP64);
+//   part 1: p1lvl = 1, one thread per (level, column) on the pointwise tiles
+//     (pw_tiles); else cb columns per block x nt threads per block (compiled:
+//     16x128, 16x256, 32x256, 64x256, 32x512, 64x512);
+//   part 2: one thread per (level, column) on the pointwise tiles, register
+//     allocation for wpe = 1 or 2 wavefronts per SIMD (2: fewer registers,
+//     with spills in FP64);
 //   sedimentation: lpc lanes per column (16, 32 or 64), each lane
 //     holding L = ceil(nlev / lpc) levels (at most 8), 4 wavefronts per block;
-//   part 3: one thread per (level, column) on the part 2 tiles (or inside the
-//     sedimentation kernel when KParams::part3_in_sed is set).
+//   part 3: one thread per (level, column) on the pointwise tiles.
 struct Shape {
   int cb, nt, lpc, wpe;
   int p1lvl;  // 1: part 1 one thread per (level, column); 0: the cb x nt block kernel
@@ -56,10 +56,9 @@ struct BatchPlan {
   Shape shape;
   const KParams* sets;  // device array, all sets (part 1 and part 2 launches)
   const int* first;     // device array, nsets + 1 entries: part 1 blocks
-  const int* first2;    // device array, nsets + 1 entries: part 2 tiles
+  const int* first2;    // device array, nsets + 1 entries: pointwise tiles (parts 1 to 3)
   int nsets;
-  int grid, grid2;      // persistent blocks of the part 1 and part 2 launches
-  int part3_sep;        // 1: part 3 in p3_part3_lvl_batch on the part 2 tiles (the sets say part3_in_sed = 0)
+  int grid, grid2;      // persistent blocks of the part 1 block kernel and the pointwise launches
   int nclass;
   SedClass cls[kMaxLevelsPerLane];
 };

@@ -51,7 +51,7 @@ template <class T> __device__ __forceinline__ T vmin(T a, T b) { return a < b ? 
 template <class T> __device__ __forceinline__ T vmax(T a, T b) { return a > b ? a : b; }
 
 // Error bits reported through the device error word.
-enum : unsigned { ERR_BAD_TEMPERATURE = 1u, ERR_SUBSTEP_LIMIT = 2u };
+enum : unsigned { ERR_BAD_TEMPERATURE = 1u, ERR_SUBSTEP_LIMIT = 2u, ERR_CHECK_VALUES = 4u };
 
 // ---------------------------------------------------------------- saturation (FP64 always)
 // MurphyKoop_svp, PSH:38-77 (with check_temperature, PSH:14-36, as a flag)

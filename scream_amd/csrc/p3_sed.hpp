@@ -72,7 +72,6 @@ __device__ __forceinline__ void sediment_run(const KParams& p, int i, int j, int
                                           int& nsub_out, unsigned& err) {
   constexpr int NF = SP == 2 ? 4 : 2;  // fields held (cloud always holds nc: its DSD rewrites it)
   const int nlev = p.nlev, kbot = nlev - 1;
-  const size_t ld = (size_t)p.ld;
   const Runtime& rt = p.rt;
   const int nfield = SP == 2 ? 4 : (SP == 1 ? 2 : (p.predictNc ? 2 : 1));
   double* fld[NF];
@@ -90,7 +89,7 @@ __device__ __forceinline__ void sediment_run(const KParams& p, int i, int j, int
   for (int t = 0; t < L; ++t) {
     const int k = kbase + t;
     const bool ok = k < nlev;
-    const size_t x = (size_t)(ok ? k : 0) * ld + i;
+    const size_t x = p.at(i, ok ? k : 0);
     const double dz = ok ? p.dz[x] : 1.0;
     const double dp = ok ? p.dpres[x] : 1.0;
     g.rho[t] = dp / dz / gravit;
@@ -99,7 +98,7 @@ __device__ __forceinline__ void sediment_run(const KParams& p, int i, int j, int
 #pragma unroll
     for (int f = 0; f < NF; ++f) q[f][t] = ok ? fld[f][x] : 0.0;
     cf[t] = ok ? cfp[x] : 1.0;
-    coef[t] = ok ? p.scratch[((size_t)coef_f * nlev + (ok ? k : 0)) * ld + i] : 0.0;
+    coef[t] = ok ? p.sc(coef_f, i, ok ? k : 0) : 0.0;
     cfl_ice[t] = (SP == 2 && ok) ? p.cld_frac_l[x] : 1.0;
     pflux[t] = 0;
     // find_bottom, P3I/p3_find_impl.hpp:52-92 (find_top gave k_top)
@@ -272,12 +271,12 @@ __device__ __forceinline__ void sediment_run(const KParams& p, int i, int j, int
   for (int t = 0; t < L; ++t) {
     const int k = kbase + t;
     if (k < nlev) {
-      const size_t x = (size_t)k * ld + i;
+      const size_t x = p.at(i, k);
       if (k <= k_bot) {
 #pragma unroll
         for (int f = 0; f < NF; ++f) fld[f][x] = q[f][t];
       }
-      if (SP == 1) p.precip_liq_flux[x + ld] = pflux[t];  // rain_sed :140-153, interface k+1
+      if (SP == 1) p.precip_liq_flux[p.atf(i, k + 1)] = pflux[t];  // rain_sed :140-153, interface k+1
     }
   }
   prt = grp_sumd<LPC>(prt_accum);  // only the lane owning kbot accumulated
@@ -290,14 +289,13 @@ __device__ __forceinline__ void sediment_run(const KParams& p, int i, int j, int
 template <int LPC, int L>
 __device__ __forceinline__ void find_tops(const KParams& p, int i, int j, bool run, int (&top)[3]) {
   const int nlev = p.nlev;
-  const size_t ld = (size_t)p.ld;
   int my[3] = {nlev, nlev, nlev};
   if (run) {
     double q[3][L];
 #pragma unroll
     for (int t = 0; t < L; ++t) {
       const int k = j * L + t;
-      const size_t x = (size_t)(k < nlev ? k : 0) * ld + i;
+      const size_t x = p.at(i, k < nlev ? k : 0);
       q[0][t] = p.qc[x];
       q[1][t] = p.qr[x];
       q[2][t] = p.qi[x];
@@ -338,14 +336,13 @@ __device__ __forceinline__ void sediment(const KParams& p, int i, int j, int k_t
 // (P3I/p3_main_impl_part3.hpp:60-199, only the fields p3_main outputs) for
 // level k of column i, after sedimentation.
 template <class R>
-__device__ __forceinline__ void part3_level(const KParams& p, int k, int i, bool active) {
+__device__ __forceinline__ void part3_level(const KParams& p, int k, int i, bool active, unsigned& err) {
   const int nlev = p.nlev;
-  const size_t ld = (size_t)p.ld;
-  const size_t x = (size_t)k * ld + i;
+  const size_t x = p.at(i, k);
   const double latvap = LatVap, latice = LatIce, inv_cp = INV_CP;
   const Runtime& rt = p.rt;
-  p.precip_ice_flux[x] = 0;
-  if (k == nlev - 1) p.precip_ice_flux[x + ld] = 0;
+  p.precip_ice_flux[p.atf(i, k)] = 0;
+  if (k == nlev - 1) p.precip_ice_flux[p.atf(i, nlev)] = 0;
   double reff_c = 10.e-6, reff_i = 25.e-6, reff_r = 500.e-6, rho_qi = 0;
   double qc = 0, nc = 0, qr = 0, nr = 0, qi = 0, ni = 0, qm = 0, bm = 0, qv = 0, th = 0;
   if (active || p.amble.on) {
@@ -358,7 +355,7 @@ __device__ __forceinline__ void part3_level(const KParams& p, int k, int i, bool
   if (active) {
     const double inv_exner = p.inv_exner[x];
     if (rt.do_ice_production) {
-      const bool t_lt = p.scratch[((size_t)SC_T * nlev + k) * ld + i] < T_homogfrz;
+      const bool t_lt = p.sc(SC_T, i, k) < T_homogfrz;
       const bool qc_ge = t_lt && qc >= QSMALL;
       const bool qr_ge = t_lt && qr >= QSMALL;
       const double Qc_nuc = qc, Qr_nuc = qr;
@@ -442,6 +439,15 @@ __device__ __forceinline__ void part3_level(const KParams& p, int k, int i, bool
     }
     if (qr < QSMALL) nr = 0;
     p.vap_liq_exchange[x] = vap_liq;
+    if (p.dbg_check) {
+      // check_values (P3I/p3_check_values_impl.hpp:25-79, called after part 3 at
+      // P3I/p3_main_impl.hpp:316-324 in debug builds) on T = th exner and qv: T outside
+      // (173, 323) K is the trap (ERR_CHECK_VALUES), qv outside (0, 40e-3) only a flag
+      const double T = th * (1 / inv_exner);
+      const int f = (!(T > 173. && T < 323.) ? 1 : 0) | (!(qv > 0. && qv < 40.e-3) ? 2 : 0);
+      if (f) atomicOr(&p.dbg_check[i], f);
+      if (f & 1) err |= ERR_CHECK_VALUES;
+    }
   } else if (p.dbg_ice_idx) {
     p.dbg_ice_idx[x] = -1;
   }
@@ -477,10 +483,8 @@ __device__ __forceinline__ void part3_level(const KParams& p, int k, int i, bool
 }
 
 // One block's work: WPB wavefronts of 64/LPC columns each, the CB = WPB * 64/LPC
-// columns starting at column bid * CB of the column set p.  Sedimentation runs
-// with LPC lanes per column; part 3, which has no vertical dependence, then
-// runs with the block's lanes over consecutive columns of a level, so that its
-// loads and stores cover whole cache-line segments.
+// columns starting at column bid * CB of the column set p, with LPC lanes per
+// column (part 3 follows in p3_part3_lvl_kernel).
 template <class R, int LPC, int L, int WPB>
 __device__ __forceinline__ void sed_block(const KParams& p, int bid) {
   constexpr int CPW = 64 / LPC;  // columns per wavefront
@@ -494,7 +498,6 @@ __device__ __forceinline__ void sed_block(const KParams& p, int bid) {
   const int i = col0 + wave * CPW + lane / LPC;
   const int nlev = p.nlev, kbot = nlev - 1;
   const int kbase = j * L;
-  const size_t ld = (size_t)p.ld;
   unsigned err = 0;
 
   // Every lane of the wavefront takes part in sediment() (its wavefront-wide
@@ -518,7 +521,7 @@ __device__ __forceinline__ void sed_block(const KParams& p, int bid) {
       const int k = kbase + t;
       if (k >= nlev) continue;
       // no rain flux above the rain's top level (the rain run stored the rest)
-      if (k < top_r) p.precip_liq_flux[(size_t)(k + 1) * ld + i] = 0;
+      if (k < top_r) p.precip_liq_flux[p.atf(i, k + 1)] = 0;
       if (k == kbot) {
         double liq = 0, ice = 0;
         if (pres_c) liq = prt_c * INV_RHO_H2O * inv_dt;
@@ -540,36 +543,33 @@ __device__ __forceinline__ void sed_block(const KParams& p, int bid) {
       }
     }
     if (j == 0) {
-      p.precip_liq_flux[i] = 0;  // the top interface
+      p.precip_liq_flux[p.atf(i, 0)] = 0;  // the top interface
       // 0 skipped after part 1, 1 skipped after part 2, 2 full step
       if (p.dbg_status) p.dbg_status[i] = status == kStatusFull ? 2 : status;
       if (p.dbg_substeps) {
         p.dbg_substeps[i] = ns_c;
-        p.dbg_substeps[(size_t)p.ncol + i] = ns_r;
-        p.dbg_substeps[(size_t)2 * p.ncol + i] = ns_i;
+        p.dbg_substeps[(size_t)p.dbg_stride + i] = ns_r;
+        p.dbg_substeps[(size_t)2 * p.dbg_stride + i] = ns_i;
       }
-    }
-  }
-  if (p.part3_in_sed) {
-    // the sedimented fields, stored above by other lanes of the block, are read below
-    __syncthreads();
-    for (int e = threadIdx.x; e < CB * nlev; e += 64 * WPB) {
-      const int k = e / CB, c = e % CB, ic = col0 + c;
-      if (ic < p.ncol) part3_level<R>(p, k, ic, p.colstat[ic] == kStatusFull);
     }
   }
   if (err) atomicOr(p.err, err);
 }
 
-// Part 3 as a kernel of its own, one thread per (level, column) on the part 2
-// tiles, after the sedimentation kernel (part3_in_sed = 0).
+// Part 3 as a kernel of its own, one thread per (level, column) on the
+// pointwise tiles, after the sedimentation kernel.
 template <class R>
-__global__ __launch_bounds__(kPart2Cols * kPart2Levels) void p3_part3_lvl_kernel(const KParams p) {
-  const int ntc = (p.ncol + kPart2Cols - 1) / kPart2Cols;
-  const int i = ((int)blockIdx.x % ntc) * kPart2Cols + (threadIdx.x & 63);
-  const int k = ((int)blockIdx.x / ntc) * kPart2Levels + (threadIdx.x >> 6);
-  if (i >= p.ncol || k >= p.nlev) return;
-  part3_level<R>(p, k, i, p.colstat[i] == kStatusFull);
+__device__ __forceinline__ void part3_tile(const KParams& p, int tile) {
+  int i, k;
+  if (!pw_point(p, tile, i, k)) return;
+  unsigned err = 0;
+  part3_level<R>(p, k, i, p.colstat[i] == kStatusFull, err);
+  if (err) atomicOr(p.err, err);
+}
+
+template <class R>
+__global__ __launch_bounds__(kTileThreads) void p3_part3_lvl_kernel(const KParams p) {
+  part3_tile<R>(p, blockIdx.x);
 }
 
 // Up to 5 levels per lane the kernel fits two wavefronts per SIMD (256 registers).

@@ -1,8 +1,8 @@
 // p3x_api.hip -- the C ABI of include/p3x.h: context, table upload, launch.
 //
 // A context owns the device copies of the lookup tables (P3LookupTables,
-// components/eamxx/src/physics/p3/p3_functions.hpp:279-290), the level-major
-// scratch the two kernels of a step share (the counterpart of the reference's
+// components/eamxx/src/physics/p3/p3_functions.hpp:279-290), the scratch the
+// kernels of a step share (the counterpart of the reference's
 // WorkspaceManager, eamxx_p3_process_interface.cpp:423-424), an error word and
 // timing events.  Nothing is allocated per call unless a larger problem than
 // any before needs more scratch.
@@ -21,6 +21,13 @@ namespace p3x_detail {
 
 int dev_fail(hipError_t e) { return e == hipErrorOutOfMemory ? P3X_ERR_NO_MEMORY : P3X_ERR_DEVICE; }
 
+int status_of_err(unsigned err) {
+  if (err & p3d::ERR_BAD_TEMPERATURE) return P3X_ERR_TEMPERATURE;
+  if (err & p3d::ERR_SUBSTEP_LIMIT) return P3X_ERR_SUBSTEPS;
+  if (err & p3d::ERR_CHECK_VALUES) return P3X_ERR_CHECK_VALUES;
+  return err ? P3X_ERR_DEVICE : P3X_OK;
+}
+
 // Default launch shape per (precision, nlev), from sweeps on MI355X at 21600x72
 // and 16384x128 in both precisions.  Part 1: 32 columns x 256 threads.  Part 2:
 // compiled for two wavefronts per SIMD (in FP64 that spills a little, which
@@ -32,8 +39,9 @@ void default_shape(int precision, int nk, p3d::Shape* sh) {
   sh->nt = 256;
   sh->wpe = 2;
   // Part 1 one thread per (level, column): part 1 + 2 take 267 instead of 317 us
-  // (FP64) and 180 instead of 221 us (mixed) at 21600 x 72.  The batched form,
-  // and any explicit cb x nt shape, use the block kernel.
+  // (FP64) and 180 instead of 221 us (mixed) at 21600 x 72.  Every form uses it
+  // unless an explicit cb x nt shape asks for the block kernel (the batch tuner
+  // tries those too).  Part 3 runs in a kernel of its own in every form.
   sh->p1lvl = 1;
   // Sedimentation lanes per column.  FP64 holds more registers per level, so it
   // moves to 32 lanes (at most 3 levels per lane at 72 levels) sooner; the mixed
@@ -93,14 +101,52 @@ std::vector<T> pack_tables(const p3x_table_data* t) {
   return v;
 }
 
-// KParams of one column set; scratch points at SC_COUNT*nk*ld doubles followed
-// by ld ints.
-void fill_params(const p3x_context* c, const p3x_runtime* runtime, const p3x_prognostic_state* ps,
-                 const p3x_diagnostic_inputs* di, const p3x_diagnostic_outputs* dout,
-                 const p3x_infrastructure* infra, const p3x_history_only* hist, int nj, int nk, int ld,
-                 int* col_status, int* sed_substeps, int* ice_table_idx, double* scratch,
-                 p3d::KParams* out) {
+int set_geometry(int nj, int nk, int layout, int ld, int ld_iface, p3d::KParams* k) {
+  if (nj < 0 || nk <= 0) return P3X_ERR_ARG;
+  k->ncol = nj;
+  k->nlev = nk;
+  k->dbg_stride = nj;
+  k->sfs = (size_t)nj * nk;
+  if (layout == P3X_LEVEL_MAJOR) {
+    const int l = ld > 0 ? ld : nj;
+    if (l < nj || (ld_iface != 0 && ld_iface != l)) return P3X_ERR_ARG;
+    k->cs = 1; k->ks = l;
+    k->fcs = 1; k->fks = l;
+    k->scs = 1; k->sks = nj;
+    k->colmajor = 0;
+    return P3X_OK;
+  }
+  if (layout == P3X_COLUMN_MAJOR) {
+    const int l = ld > 0 ? ld : nk;
+    const int lf = ld_iface > 0 ? ld_iface : std::max(l, nk + 1);
+    if (l < nk || lf < nk + 1) return P3X_ERR_ARG;
+    // the column-major pointwise tiles index points with 32-bit arithmetic
+    if ((long long)nj * nk >= (1ll << 31)) return P3X_ERR_ARG;
+    k->cs = l; k->ks = 1;
+    k->fcs = lf; k->fks = 1;
+    k->scs = nk; k->sks = 1;
+    k->colmajor = 1;
+    return P3X_OK;
+  }
+  return P3X_ERR_ARG;
+}
+
+size_t level_span(const p3d::KParams& k) {
+  if (k.ncol <= 0) return 0;
+  return (size_t)(k.ncol - 1) * k.cs + (size_t)(k.nlev - 1) * k.ks + 1;
+}
+
+// KParams of one column set; scratch points at SC_COUNT*nj*nk doubles followed
+// by nj ints.
+int fill_params(const p3x_context* c, const p3x_runtime* runtime, const p3x_prognostic_state* ps,
+                const p3x_diagnostic_inputs* di, const p3x_diagnostic_outputs* dout,
+                const p3x_infrastructure* infra, const p3x_history_only* hist, int nj, int nk, int layout,
+                int ld, int ld_iface, int* col_status, int* sed_substeps, int* ice_table_idx, double* scratch,
+                p3d::KParams* out) {
   p3d::KParams& k = *out;
+  k = p3d::KParams{};
+  const int st = set_geometry(nj, nk, layout, ld, ld_iface, &k);
+  if (st != P3X_OK) return st;
   k.qc = ps->qc; k.nc = ps->nc; k.qr = ps->qr; k.nr = ps->nr; k.qi = ps->qi;
   k.qm = ps->qm; k.ni = ps->ni; k.bm = ps->bm; k.qv = ps->qv; k.th = ps->th;
   k.nc_nuceat_tend = di->nc_nuceat_tend; k.nccn = di->nccn; k.ni_activated = di->ni_activated;
@@ -115,7 +161,7 @@ void fill_params(const p3x_context* c, const p3x_runtime* runtime, const p3x_pro
   k.liq_ice_exchange = hist->liq_ice_exchange; k.vap_liq_exchange = hist->vap_liq_exchange;
   k.vap_ice_exchange = hist->vap_ice_exchange;
   k.scratch = scratch;
-  k.colstat = (int*)(scratch + (size_t)p3d::SC_COUNT * nk * ld);
+  k.colstat = scratch ? (int*)(scratch + (size_t)p3d::SC_COUNT * k.sfs) : nullptr;
   k.err = c->d_err;
   k.dbg_status = col_status;
   k.dbg_substeps = sed_substeps;
@@ -124,7 +170,6 @@ void fill_params(const p3x_context* c, const p3x_runtime* runtime, const p3x_pro
   k.tab.f = table_set(c->d_tab_f);
   static_assert(sizeof(p3d::Runtime) == sizeof(p3x_runtime), "runtime layouts differ");
   std::memcpy(&k.rt, runtime, sizeof(k.rt));
-  k.ncol = nj; k.nlev = nk; k.ld = ld;
   k.dt = infra->dt;
   k.predictNc = infra->predictNc ? 1 : 0;
   k.prescribedCCN = infra->prescribedCCN ? 1 : 0;
@@ -134,7 +179,7 @@ void fill_params(const p3x_context* c, const p3x_runtime* runtime, const p3x_pro
   unsigned no_err = 0;
   k.e_melt = p3d::mk_svp(p3c::Tmelt, false, no_err);
   k.amble = p3d::Amble{};  // no fused pre/post-amble
-  k.part3_in_sed = 1;
+  return P3X_OK;
 }
 
 // Every field pointer of a column set is non-null.
@@ -153,10 +198,10 @@ bool fields_present(const p3x_prognostic_state* ps, const p3x_diagnostic_inputs*
   return true;
 }
 
-// Bytes of scratch one column set needs (SC_COUNT level-major fields, then ld
-// column statuses), rounded up to 256.
-size_t scratch_bytes_for(int nk, int ld) {
-  const size_t b = (size_t)p3d::SC_COUNT * nk * ld * sizeof(double) + (size_t)ld * sizeof(int);
+// Bytes of scratch one column set needs (SC_COUNT dense fields of nj x nk, then
+// nj column statuses), rounded up to 256.
+size_t scratch_bytes_for(int nj, int nk) {
+  const size_t b = (size_t)p3d::SC_COUNT * nk * nj * sizeof(double) + (size_t)nj * sizeof(int);
   return (b + 255) & ~(size_t)255;
 }
 
@@ -204,6 +249,8 @@ extern "C" const char* p3x_strerror(int st) {
     case P3X_ERR_TEMPERATURE: return "temperature <= 0 or NaN reached the saturation functions";
     case P3X_ERR_NO_MEMORY: return "device memory allocation failed";
     case P3X_ERR_SUBSTEPS: return "sedimentation needed more than 100000 CFL substeps (corrupt input)";
+    case P3X_ERR_BUDGET: return "the member does not fit the server's HBM budget";
+    case P3X_ERR_CHECK_VALUES: return "check_values: a temperature outside (173, 323) K after the step";
     default: return "unknown P3 status";
   }
 }
@@ -264,7 +311,7 @@ int p3x_detail::grow_scratch(p3x_context* c, size_t need) {
 
 extern "C" int p3x_reserve(p3x_context* c, int ld, int nk) {
   if (!c || ld <= 0 || nk <= 0 || nk > 64 * p3d::kMaxLevelsPerLane) return P3X_ERR_ARG;
-  return grow_scratch(c, scratch_bytes_for(nk, ld));
+  return grow_scratch(c, scratch_bytes_for(ld, nk));
 }
 
 extern "C" int p3x_sync(p3x_context* c) {
@@ -278,7 +325,7 @@ extern "C" int p3x_sync(p3x_context* c) {
   if (e != hipSuccess) return P3X_ERR_DEVICE;
   if (err) {
     (void)hipMemset(c->d_err, 0, sizeof(unsigned));
-    return (err & p3d::ERR_BAD_TEMPERATURE) ? P3X_ERR_TEMPERATURE : P3X_ERR_SUBSTEPS;
+    return status_of_err(err);
   }
   return P3X_OK;
 }
@@ -299,8 +346,10 @@ int p3x_detail::main_impl(p3x_context* c, const p3x_runtime* runtime, const p3x_
   p3x_launch o;
   if (opts) o = *opts;
   else p3x_launch_defaults(&o);
-  const int ld = o.ld > 0 ? o.ld : nj;
-  if (ld < nj) return P3X_ERR_ARG;
+  {
+    p3d::KParams g;
+    if (set_geometry(nj, nk, o.layout, o.ld, o.ld_iface, &g) != P3X_OK) return P3X_ERR_ARG;
+  }
   if (o.precision != P3X_FP64 && o.precision != P3X_MIXED) return P3X_ERR_ARG;
   if (nk > 64 * p3d::kMaxLevelsPerLane) return P3X_ERR_ARG;
   p3d::Shape sh;
@@ -314,19 +363,18 @@ int p3x_detail::main_impl(p3x_context* c, const p3x_runtime* runtime, const p3x_
   if (!fields_present(ps, di, dout, hist)) return P3X_ERR_ARG;
 
   {
-    const int st = grow_scratch(c, scratch_bytes_for(nk, ld));
+    const int st = grow_scratch(c, scratch_bytes_for(nj, nk));
     if (st != P3X_OK) return st;
   }
 
   p3d::KParams k;
-  fill_params(c, runtime, ps, di, dout, infra, hist, nj, nk, ld, o.col_status, o.sed_substeps,
-              o.ice_table_idx, c->d_scratch, &k);
+  {
+    const int st = fill_params(c, runtime, ps, di, dout, infra, hist, nj, nk, o.layout, o.ld, o.ld_iface,
+                               o.col_status, o.sed_substeps, o.ice_table_idx, c->d_scratch, &k);
+    if (st != P3X_OK) return st;
+  }
+  k.dbg_check = o.check_values;
   if (amble) k.amble = *amble;
-  // part 3 in a kernel of its own, one thread per (level, column): at 16384 x 128
-  // the sedimentation + part 3 time fell 9 % (FP64) and 17 % (mixed), at
-  // 21600 x 72 by 1-4 %.  The batched and served forms keep it in the
-  // sedimentation kernel.
-  k.part3_in_sed = 0;
 
   // Every call on this context shares its scratch: a call on another stream than
   // the previous one waits on the device for the previous one to end.
@@ -390,7 +438,7 @@ int layout_batch(const p3d::KParams* sets, int n, const p3d::Shape& sh, Layout* 
   L.first2.assign(n + 1, 0);
   for (int i = 0; i < n; ++i) {
     L.first[i + 1] = L.first[i] + (sets[i].ncol + sh.cb - 1) / sh.cb;
-    L.first2[i + 1] = L.first2[i] + p3d::part2_tiles(sets[i].ncol, sets[i].nlev);
+    L.first2[i + 1] = L.first2[i] + p3d::pw_tiles(sets[i]);
   }
   const int cols_sed = p3d::kSedWaves * (64 / sh.lpc);
   std::vector<int> cls_of(n);
@@ -502,7 +550,6 @@ int build_plan(p3x_batch* b) {
   b->plan.first = b->d_first;
   b->plan.first2 = b->d_first + (n + 1);
   b->plan.nsets = n;
-  b->plan.part3_sep = 1;
   b->plan.nclass = (int)lay.cls.size();
   for (int ci = 0; ci < b->plan.nclass; ++ci) {
     const Layout::Cls& lc = lay.cls[ci];
@@ -573,13 +620,14 @@ extern "C" int p3x_batch_create(p3x_context* c, const p3x_runtime* runtime, cons
   size_t total = 0;
   for (int i = 0; i < nsets; ++i) {
     const p3x_column_set& s = sets[i];
-    const int ld = s.ld > 0 ? s.ld : s.nj;
-    if (s.nj <= 0 || s.nk <= 0 || ld < s.nj || s.nk > 64 * p3d::kMaxLevelsPerLane) return P3X_ERR_ARG;
+    p3d::KParams g;
+    if (s.nj <= 0 || s.nk > 64 * p3d::kMaxLevelsPerLane) return P3X_ERR_ARG;
+    if (set_geometry(s.nj, s.nk, s.layout, s.ld, s.ld_iface, &g) != P3X_OK) return P3X_ERR_ARG;
     if (!(s.infrastructure.dt > 0)) return P3X_ERR_ARG;
     if (!fields_present(&s.prognostic_state, &s.diagnostic_inputs, &s.diagnostic_outputs, &s.history_only))
       return P3X_ERR_ARG;
     offs[i] = total;
-    total += scratch_bytes_for(s.nk, ld);
+    total += scratch_bytes_for(s.nj, s.nk);
   }
   p3x_batch* b = new p3x_batch();
   b->ctx = c;
@@ -589,11 +637,10 @@ extern "C" int p3x_batch_create(p3x_context* c, const p3x_runtime* runtime, cons
     b->sets.resize(nsets);
     for (int i = 0; i < nsets; ++i) {
       const p3x_column_set& s = sets[i];
-      const int ld = s.ld > 0 ? s.ld : s.nj;
       fill_params(c, runtime, &s.prognostic_state, &s.diagnostic_inputs, &s.diagnostic_outputs,
-                  &s.infrastructure, &s.history_only, s.nj, s.nk, ld, s.col_status, s.sed_substeps,
-                  s.ice_table_idx, (double*)((char*)b->d_scratch + offs[i]), &b->sets[i]);
-      b->sets[i].part3_in_sed = 0;  // part 3 in its own launch after sedimentation
+                  &s.infrastructure, &s.history_only, s.nj, s.nk, s.layout, s.ld, s.ld_iface, s.col_status,
+                  s.sed_substeps, s.ice_table_idx, (double*)((char*)b->d_scratch + offs[i]), &b->sets[i]);
+      b->sets[i].dbg_check = s.check_values;
     }
     e = hipMalloc((void**)&b->d_sets, nsets * sizeof(p3d::KParams));
   }
@@ -631,7 +678,7 @@ extern "C" int p3x_batch_tune(p3x_batch* b, void* stream, int reps, p3x_batch_op
   for (const auto& k : b->sets) {
     const size_t n0 = arrs.size();
     inout_arrays(k, &arrs);
-    for (size_t a = n0; a < arrs.size(); ++a) bytes.push_back((size_t)k.nlev * k.ld * sizeof(double));
+    for (size_t a = n0; a < arrs.size(); ++a) bytes.push_back(level_span(k) * sizeof(double));
   }
   size_t total = 0;
   for (size_t x : bytes) total += x;

@@ -31,26 +31,33 @@ struct p3x_context {
 namespace p3x_detail {
 
 int dev_fail(hipError_t e);
+// The P3X status of a nonzero device error word.
+int status_of_err(unsigned err);
 // Default launch shape per (precision, nlev).
 void default_shape(int precision, int nk, p3d::Shape* sh);
-// KParams of one column set; scratch points at SC_COUNT*nk*ld doubles followed by ld ints.
 // Grows the context's scratch to at least need bytes (P3X_OK or an error status).
 int grow_scratch(p3x_context* c, size_t need);
+// Layout strides of a column set in k (P3X_ERR_ARG for an invalid layout or stride).
+int set_geometry(int nj, int nk, int layout, int ld, int ld_iface, p3d::KParams* k);
+// Elements a level field of k spans from its first element to its last.
+size_t level_span(const p3d::KParams& k);
 // p3x_main with (amble != NULL) the P3Microphysics pre/post-amble fused into its kernels.
 int main_impl(p3x_context* c, const p3x_runtime* runtime, const p3x_prognostic_state* ps,
               const p3x_diagnostic_inputs* di, const p3x_diagnostic_outputs* dout,
               const p3x_infrastructure* infra, const p3x_history_only* hist, int nj, int nk,
               const p3x_launch* opts, double* elapsed_us, const p3d::Amble* amble);
-void fill_params(const p3x_context* c, const p3x_runtime* runtime, const p3x_prognostic_state* ps,
-                 const p3x_diagnostic_inputs* di, const p3x_diagnostic_outputs* dout,
-                 const p3x_infrastructure* infra, const p3x_history_only* hist, int nj, int nk, int ld,
-                 int* col_status, int* sed_substeps, int* ice_table_idx, double* scratch,
-                 p3d::KParams* out);
+// KParams of one column set; scratch (NULL: set later) points at SC_COUNT*nj*nk
+// doubles followed by nj ints.  P3X_ERR_ARG for an invalid layout.
+int fill_params(const p3x_context* c, const p3x_runtime* runtime, const p3x_prognostic_state* ps,
+                const p3x_diagnostic_inputs* di, const p3x_diagnostic_outputs* dout,
+                const p3x_infrastructure* infra, const p3x_history_only* hist, int nj, int nk, int layout,
+                int ld, int ld_iface, int* col_status, int* sed_substeps, int* ice_table_idx, double* scratch,
+                p3d::KParams* out);
 // Every field pointer of a column set is non-null.
 bool fields_present(const p3x_prognostic_state* ps, const p3x_diagnostic_inputs* di,
                     const p3x_diagnostic_outputs* dout, const p3x_history_only* hist);
-// Bytes of scratch one column set needs, rounded up to 256.
-size_t scratch_bytes_for(int nk, int ld);
+// Bytes of scratch one column set of nj columns x nk levels needs, rounded up to 256.
+size_t scratch_bytes_for(int nj, int nk);
 
 // Launch shape for a batch whose deepest set has nk_max levels: the defaults,
 // overridden by the nonzero options.  P3X_ERR_ARG if not compiled in.

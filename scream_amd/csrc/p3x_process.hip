@@ -48,6 +48,7 @@ struct BudgetParams {
   p3x_budget_fields f;
   double *mass, *energy;
   int ncol, nlev;
+  int cs, ks;  // element (i, k) at i*cs + k*ks
 };
 
 // One thread per column, levels in order: the sums round like the oracle's.
@@ -58,7 +59,7 @@ __global__ __launch_bounds__(256) void p3_column_budget_kernel(const BudgetParam
   const double g = p3c::gravit, cp = p3c::CP, lv = p3c::LatVap, lf = p3c::LatIce;
   double m = 0, e = 0;
   for (int k = 0; k < b.nlev; ++k) {
-    const size_t x = (size_t)k * b.ncol + i;
+    const size_t x = (size_t)i * b.cs + (size_t)k * b.ks;
     const double dp = f.pseudo_density[x];
     const double qv = f.qv[x], qc = f.qc[x], qi = f.qi[x], qr = f.qr[x];
     m += (qv + qc + qi + qr) * dp / g;
@@ -95,8 +96,10 @@ extern "C" int p3x_column_budget(p3x_context* c, const p3x_budget_fields* f, int
                                  double* energy, void* stream) {
   if (!c || !f || !mass || !energy || ncol < 0 || nlev <= 0) return P3X_ERR_ARG;
   if (!f->pseudo_density || !f->T_mid || !f->qv || !f->qc || !f->qi || !f->qr) return P3X_ERR_ARG;
+  p3d::KParams g;
+  if (set_geometry(ncol, nlev, f->layout, f->ld, 0, &g) != P3X_OK) return P3X_ERR_ARG;
   if (ncol == 0) return P3X_OK;
-  const BudgetParams b{*f, mass, energy, ncol, nlev};
+  const BudgetParams b{*f, mass, energy, ncol, nlev, g.cs, g.ks};
   hipLaunchKernelGGL(p3_column_budget_kernel, dim3((ncol + 255) / 256), dim3(256), 0, (hipStream_t)stream, b);
   return hipGetLastError() == hipSuccess ? P3X_OK : P3X_ERR_DEVICE;
 }
@@ -132,6 +135,7 @@ struct p3x_process {
   p3x_context* ctx = nullptr;
   p3x_runtime rt{};
   int ncol = 0, nlev = 0, predictNc = 1, prescribedCCN = 0, it = 0;
+  int layout = P3X_LEVEL_MAJOR, ld = 0, ld_iface = 0;  // of the fields and of the local arrays
   double* mem = nullptr;  // every process-local array below
   double *inv_exner, *th_atm, *cld_frac_l, *cld_frac_i, *dz, *qv2qi_depos_tend, *rho_qi;
   double *precip_liq_flux, *precip_ice_flux, *precip_liq_surf, *precip_ice_surf, *unused;
@@ -139,8 +143,15 @@ struct p3x_process {
 
 extern "C" int p3x_process_create(p3x_context* c, const p3x_runtime* rt, int ncol, int nlev, int predictNc,
                                   int prescribedCCN, p3x_process** out) {
+  return p3x_process_create_layout(c, rt, ncol, nlev, predictNc, prescribedCCN, P3X_LEVEL_MAJOR, 0, 0, out);
+}
+
+extern "C" int p3x_process_create_layout(p3x_context* c, const p3x_runtime* rt, int ncol, int nlev, int predictNc,
+                                         int prescribedCCN, int layout, int ld, int ld_iface, p3x_process** out) {
   if (!c || !rt || !out || ncol <= 0 || nlev <= 0 || nlev > 64 * p3d::kMaxLevelsPerLane) return P3X_ERR_ARG;
   *out = nullptr;
+  p3d::KParams g;
+  if (set_geometry(ncol, nlev, layout, ld, ld_iface, &g) != P3X_OK) return P3X_ERR_ARG;
   p3x_process* p = new p3x_process();
   p->ctx = c;
   p->rt = *rt;
@@ -148,7 +159,12 @@ extern "C" int p3x_process_create(p3x_context* c, const p3x_runtime* rt, int nco
   p->nlev = nlev;
   p->predictNc = predictNc ? 1 : 0;
   p->prescribedCCN = prescribedCCN ? 1 : 0;
-  const size_t lev = (size_t)ncol * nlev, iface = (size_t)ncol * (nlev + 1);
+  // the local arrays have the fields' strides, so that one set of strides indexes both
+  p->layout = layout;
+  p->ld = layout == P3X_LEVEL_MAJOR ? g.ks : g.cs;
+  p->ld_iface = layout == P3X_LEVEL_MAJOR ? 0 : g.fcs;
+  const size_t lev = layout == P3X_LEVEL_MAJOR ? (size_t)nlev * g.ks : (size_t)ncol * g.cs;
+  const size_t iface = layout == P3X_LEVEL_MAJOR ? (size_t)(nlev + 1) * g.fks : (size_t)ncol * g.fcs;
   const size_t total = 8 * lev + 2 * iface + 2 * (size_t)ncol;
   const hipError_t e = hipMalloc((void**)&p->mem, total * sizeof(double));
   if (e != hipSuccess) {
@@ -196,7 +212,13 @@ extern "C" int p3x_process_run(p3x_process* p, const p3x_atm_fields* f, double d
   p3x_launch o;
   if (opts) o = *opts;
   else p3x_launch_defaults(&o);
-  if (o.ld != 0 && o.ld != p->ncol) return P3X_ERR_ARG;  // the process's arrays are dense
+  // the layout is the process's own (its local arrays were allocated in it)
+  if ((o.layout != P3X_LEVEL_MAJOR && o.layout != p->layout) || (o.ld != 0 && o.ld != p->ld) ||
+      (o.ld_iface != 0 && o.ld_iface != p->ld_iface))
+    return P3X_ERR_ARG;
+  o.layout = p->layout;
+  o.ld = p->ld;
+  o.ld_iface = p->ld_iface;
   hipStream_t s = (hipStream_t)o.stream;
 
   // p3_main on the process's views (eamxx_p3_process_interface.cpp:284-337); the

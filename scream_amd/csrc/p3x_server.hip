@@ -3,18 +3,20 @@
 //
 // One scheduler thread owns every device call after creation.  It repeatedly
 //   1. retires finished slots (their completion event has fired): each request
-//      of the batch gets its latency and the batch's error status, its member
+//      of the batch gets its latency and its own set's error status, its member
 //      becomes free;
 //   2. fills every free slot with a new batch: the queued requests, in arrival
 //      order, whose member is neither in flight nor already in this batch and
-//      whose scratch still fits the slot's share of the budget, up to
-//      max_batch of them; the batch's set descriptors and block offsets go
-//      through the slot's pinned staging buffer to its device buffers
-//      (asynchronous copies on the slot's stream), followed by the two launches
-//      of the batched kernels, a read-back of the slot's error word and an
-//      event.
+//      whose scratch still fits the slot's share of the scratch budget, up to
+//      max_batch of them; the batch's set descriptors, block offsets and one
+//      zeroed error word per set go through the slot's pinned staging buffer to
+//      its device buffer (one asynchronous copy on the slot's stream), followed
+//      by the launches of the batched kernels, a read-back of the error words
+//      and an event.
 // Slots are independent non-blocking streams, so batches admitted while
-// others run overlap on the device.
+// others run overlap on the device.  Members may register at any time; each
+// is admitted against the HBM budget (the bytes its resident arrays span plus
+// the slots' scratch).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -49,11 +51,10 @@ struct Slot {
   hipEvent_t done = nullptr;
   char* d_arena = nullptr;  // scratch of the batch's sets
   size_t arena_bytes = 0;
-  char* d_desc = nullptr;   // set descriptors and block offsets
+  char* d_desc = nullptr;   // set descriptors, block offsets and the sets' error words
   char* h_stage = nullptr;  // pinned staging of d_desc
   size_t desc_bytes = 0;
-  unsigned* d_err = nullptr;
-  unsigned* h_err = nullptr;  // pinned
+  unsigned* h_err = nullptr;  // pinned, max_batch words: the error word of each set
   bool busy = false;
   std::vector<Request> reqs;
 };
@@ -69,10 +70,13 @@ struct p3x_server {
   int occ_sed[p3d::kMaxLevelsPerLane + 1] = {};
   struct Member {
     p3d::KParams k;
-    size_t scratch;
+    size_t scratch;   // scratch bytes of one step
+    size_t resident;  // bytes its device arrays span
     bool busy;
   };
   std::vector<Member> members;
+  double resident_bytes = 0;  // of all members
+  double scratch_total = 0;   // the slots' arenas
   std::deque<Request> pending;
   std::vector<Slot> slots;
   std::deque<p3x_completion> done;
@@ -90,43 +94,59 @@ namespace {
 
 size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
-// Bytes of descriptors a batch of n sets needs in a slot.
+// Bytes of descriptors a batch of n sets needs in a slot: [sets][sed sets],
+// [first], [first2], [sed_first], [error words], each 256-aligned.
 size_t desc_bytes_for(int n) {
   return align256(2 * (size_t)n * sizeof(p3d::KParams)) + 2 * align256((size_t)(n + 1) * sizeof(int)) +
-         align256((size_t)(n + p3d::kMaxLevelsPerLane) * sizeof(int));
+         align256((size_t)(n + p3d::kMaxLevelsPerLane) * sizeof(int)) + align256((size_t)n * sizeof(unsigned));
+}
+
+// Bytes the device arrays of a member span (every field of its column set).
+size_t member_bytes(const p3d::KParams& k) {
+  const size_t lev = level_span(k);
+  const size_t iface = (size_t)(k.ncol - 1) * k.fcs + (size_t)k.nlev * k.fks + 1;
+  // 10 state + 13 input + 7 per-level output + 3 exchange fields, 2 fluxes, 2 surface fields
+  size_t b = (33 * lev + 2 * iface + 2 * (size_t)k.ncol) * sizeof(double);
+  if (k.dbg_ice_idx) b += lev * sizeof(int);
+  if (k.dbg_status) b += (size_t)k.ncol * sizeof(int);
+  if (k.dbg_substeps) b += 3 * (size_t)k.dbg_stride * sizeof(int);
+  return b;
 }
 
 // Enqueue the batch of slot s (srv->mu held).  Returns a P3X status.
 int launch_slot(p3x_server* srv, Slot& sl) {
   const int n = (int)sl.reqs.size();
   std::vector<p3d::KParams> sets(n);
+  // staging layout: [sets][sed sets][first][first2][sed_first][error words]
+  const size_t o_sets = 0, o_sed = (size_t)n * sizeof(p3d::KParams);
+  const size_t o_first = align256(2 * (size_t)n * sizeof(p3d::KParams));
+  const size_t o_first2 = o_first + align256((size_t)(n + 1) * sizeof(int));
+  const size_t o_sfirst = o_first2 + align256((size_t)(n + 1) * sizeof(int));
+  const size_t o_err = o_sfirst + align256((size_t)(n + p3d::kMaxLevelsPerLane) * sizeof(int));
+  unsigned* d_err = (unsigned*)(sl.d_desc + o_err);
   size_t off = 0;
   for (int i = 0; i < n; ++i) {
     const Request& r = sl.reqs[i];
     p3d::KParams k = srv->members[r.member].k;
     k.dt = r.dt;
     k.scratch = (double*)(sl.d_arena + off);
-    k.colstat = (int*)((char*)k.scratch + (size_t)p3d::SC_COUNT * k.nlev * k.ld * sizeof(double));
-    k.err = sl.d_err;
+    k.colstat = (int*)(k.scratch + (size_t)p3d::SC_COUNT * k.sfs);
+    k.err = d_err + i;  // one error word per set: a failing step is charged to its own request
     off += srv->members[r.member].scratch;
     sets[i] = k;
   }
   Layout lay;
   int st = layout_batch(sets.data(), n, srv->shape, &lay);
   if (st != P3X_OK) return st;
-  // staging layout: [sets][sed sets][first][first2][sed_first]
   char* h = sl.h_stage;
-  const size_t o_sets = 0, o_sed = (size_t)n * sizeof(p3d::KParams);
-  const size_t o_first = align256(2 * (size_t)n * sizeof(p3d::KParams));
-  const size_t o_first2 = o_first + align256((size_t)(n + 1) * sizeof(int));
-  const size_t o_sfirst = o_first2 + align256((size_t)(n + 1) * sizeof(int));
   std::memcpy(h + o_sets, sets.data(), n * sizeof(p3d::KParams));
   for (int i = 0; i < n; ++i)
     std::memcpy(h + o_sed + i * sizeof(p3d::KParams), &sets[lay.sed_order[i]], sizeof(p3d::KParams));
   std::memcpy(h + o_first, lay.first.data(), lay.first.size() * sizeof(int));
   std::memcpy(h + o_first2, lay.first2.data(), lay.first2.size() * sizeof(int));
   std::memcpy(h + o_sfirst, lay.sed_first.data(), lay.sed_first.size() * sizeof(int));
-  const size_t bytes = o_sfirst + lay.sed_first.size() * sizeof(int);
+  std::memset(h + o_err, 0, (size_t)n * sizeof(unsigned));  // the error words start at zero
+  const size_t bytes = o_err + (size_t)n * sizeof(unsigned);
 
   p3d::BatchPlan plan{};
   plan.shape = srv->shape;
@@ -134,7 +154,6 @@ int launch_slot(p3x_server* srv, Slot& sl) {
   plan.first = (const int*)(sl.d_desc + o_first);
   plan.first2 = (const int*)(sl.d_desc + o_first2);
   plan.nsets = n;
-  plan.part3_sep = 1;
   plan.grid = grid_for(srv->ctx->ncu, 0, lay.first[n], srv->occ1);
   plan.grid2 = grid_for(srv->ctx->ncu, 0, lay.first2[n], srv->occ2);
   plan.nclass = (int)lay.cls.size();
@@ -148,9 +167,9 @@ int launch_slot(p3x_server* srv, Slot& sl) {
     sc.grid = grid_for(srv->ctx->ncu, 0, lc.total, srv->occ_sed[lc.L]);
   }
   hipError_t e = hipMemcpyAsync(sl.d_desc, h, bytes, hipMemcpyHostToDevice, sl.stream);
-  if (e == hipSuccess) e = hipMemsetAsync(sl.d_err, 0, sizeof(unsigned), sl.stream);
   if (e == hipSuccess) e = launch_plan(srv->opt.precision, plan, sl.stream);
-  if (e == hipSuccess) e = hipMemcpyAsync(sl.h_err, sl.d_err, sizeof(unsigned), hipMemcpyDeviceToHost, sl.stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(sl.h_err, d_err, (size_t)n * sizeof(unsigned), hipMemcpyDeviceToHost, sl.stream);
   if (e == hipSuccess) e = hipEventRecord(sl.done, sl.stream);
   if (e == hipErrorInvalidValue) return P3X_ERR_ARG;
   return e == hipSuccess ? P3X_OK : P3X_ERR_DEVICE;
@@ -164,10 +183,11 @@ bool retire(p3x_server* srv) {
     const hipError_t q = hipEventQuery(sl.done);
     if (q == hipErrorNotReady) continue;
     const double t = now_us();
-    int st = P3X_OK;
-    if (q != hipSuccess) st = P3X_ERR_DEVICE;
-    else if (*sl.h_err) st = (*sl.h_err & p3d::ERR_BAD_TEMPERATURE) ? P3X_ERR_TEMPERATURE : P3X_ERR_SUBSTEPS;
-    for (const Request& r : sl.reqs) {
+    for (size_t i = 0; i < sl.reqs.size(); ++i) {
+      const Request& r = sl.reqs[i];
+      int st = P3X_OK;
+      if (q != hipSuccess) st = P3X_ERR_DEVICE;
+      else if (sl.h_err[i]) st = status_of_err(sl.h_err[i]);
       p3x_completion c{r.tag, r.member, st, t - r.t_submit};
       srv->done.push_back(c);
       srv->latencies.push_back(c.latency_us);
@@ -253,7 +273,6 @@ void free_server(p3x_server* s) {
     if (sl.done) (void)hipEventDestroy(sl.done);
     (void)hipFree(sl.d_arena);
     (void)hipFree(sl.d_desc);
-    (void)hipFree(sl.d_err);
     (void)hipHostFree(sl.h_stage);
     (void)hipHostFree(sl.h_err);
   }
@@ -269,6 +288,7 @@ extern "C" void p3x_server_defaults(p3x_server_options* o) {
   o->slots = 4;
   o->max_batch = 32;
   o->scratch_budget_bytes = 8.0 * (1 << 30);
+  o->hbm_budget_bytes = 200e9;
 }
 
 extern "C" int p3x_server_create(p3x_context* c, const p3x_runtime* rt, const p3x_server_options* opts,
@@ -282,6 +302,7 @@ extern "C" int p3x_server_create(p3x_context* c, const p3x_runtime* rt, const p3
     if (opts->slots > 0) o.slots = opts->slots;
     if (opts->max_batch > 0) o.max_batch = opts->max_batch;
     if (opts->scratch_budget_bytes > 0) o.scratch_budget_bytes = opts->scratch_budget_bytes;
+    if (opts->hbm_budget_bytes > 0) o.hbm_budget_bytes = opts->hbm_budget_bytes;
     o.columns_per_block = opts->columns_per_block;
     o.threads_per_block = opts->threads_per_block;
     o.sed_lanes_per_column = opts->sed_lanes_per_column;
@@ -289,6 +310,7 @@ extern "C" int p3x_server_create(p3x_context* c, const p3x_runtime* rt, const p3
   }
   if (o.precision != P3X_FP64 && o.precision != P3X_MIXED) return P3X_ERR_ARG;
   if (o.slots > 64 || o.max_batch > 4096) return P3X_ERR_ARG;
+  if (o.scratch_budget_bytes > o.hbm_budget_bytes) return P3X_ERR_BUDGET;
   p3x_server* s = new p3x_server();
   s->ctx = c;
   s->rt = *rt;
@@ -313,6 +335,7 @@ extern "C" int p3x_server_create(p3x_context* c, const p3x_runtime* rt, const p3
   s->occ_sed[7] = s->occ_sed[8];
   s->slots.resize(o.slots);
   const size_t arena = (size_t)(o.scratch_budget_bytes / o.slots) & ~(size_t)255;
+  s->scratch_total = (double)arena * o.slots;
   for (Slot& sl : s->slots) {
     if (e != hipSuccess) break;
     sl.arena_bytes = arena;
@@ -321,9 +344,9 @@ extern "C" int p3x_server_create(p3x_context* c, const p3x_runtime* rt, const p3
     if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
     if (e == hipSuccess) e = hipMalloc((void**)&sl.d_arena, arena);
     if (e == hipSuccess) e = hipMalloc((void**)&sl.d_desc, sl.desc_bytes);
-    if (e == hipSuccess) e = hipMalloc((void**)&sl.d_err, sizeof(unsigned));
     if (e == hipSuccess) e = hipHostMalloc((void**)&sl.h_stage, sl.desc_bytes, hipHostMallocDefault);
-    if (e == hipSuccess) e = hipHostMalloc((void**)&sl.h_err, sizeof(unsigned), hipHostMallocDefault);
+    if (e == hipSuccess)
+      e = hipHostMalloc((void**)&sl.h_err, (size_t)o.max_batch * sizeof(unsigned), hipHostMallocDefault);
   }
   if (e != hipSuccess) {
     free_server(s);
@@ -335,25 +358,30 @@ extern "C" int p3x_server_create(p3x_context* c, const p3x_runtime* rt, const p3
 
 extern "C" int p3x_server_add_member(p3x_server* s, const p3x_column_set* set, int* member) {
   if (!s || !set || !member) return P3X_ERR_ARG;
-  std::lock_guard<std::mutex> lk(s->mu);
-  if (s->started) return P3X_ERR_ARG;  // members register before the first request
-  const int ld = set->ld > 0 ? set->ld : set->nj;
-  if (set->nj <= 0 || set->nk <= 0 || ld < set->nj) return P3X_ERR_ARG;
+  if (set->nj <= 0 || set->nk <= 0) return P3X_ERR_ARG;
   if (p3d::sed_levels_per_lane(set->nk, s->shape.lpc) == 0) return P3X_ERR_ARG;
   if (!fields_present(&set->prognostic_state, &set->diagnostic_inputs, &set->diagnostic_outputs,
                       &set->history_only))
     return P3X_ERR_ARG;
-  const size_t need = scratch_bytes_for(set->nk, ld);
-  if (need > s->slots[0].arena_bytes) return P3X_ERR_ARG;
+  const size_t need = scratch_bytes_for(set->nj, set->nk);
+  if (need > s->slots[0].arena_bytes) return P3X_ERR_BUDGET;  // its scratch can never fit a slot
   p3x_server::Member m;
   p3x_infrastructure inf = set->infrastructure;
   inf.dt = 1.0;  // replaced per request
-  fill_params(s->ctx, &s->rt, &set->prognostic_state, &set->diagnostic_inputs, &set->diagnostic_outputs, &inf,
-              &set->history_only, set->nj, set->nk, ld, set->col_status, set->sed_substeps, set->ice_table_idx,
-              nullptr, &m.k);
-  m.k.part3_in_sed = 0;  // part 3 in its own launch after sedimentation
+  const int st = fill_params(s->ctx, &s->rt, &set->prognostic_state, &set->diagnostic_inputs,
+                             &set->diagnostic_outputs, &inf, &set->history_only, set->nj, set->nk, set->layout,
+                             set->ld, set->ld_iface, set->col_status, set->sed_substeps, set->ice_table_idx,
+                             nullptr, &m.k);
+  if (st != P3X_OK) return st;
+  m.k.dbg_check = set->check_values;
   m.scratch = need;
+  m.resident = member_bytes(m.k);
   m.busy = false;
+  // registration may happen while the scheduler serves others: the members are
+  // only touched under the lock
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (s->resident_bytes + (double)m.resident + s->scratch_total > s->opt.hbm_budget_bytes) return P3X_ERR_BUDGET;
+  s->resident_bytes += (double)m.resident;
   s->members.push_back(m);
   *member = (int)s->members.size() - 1;
   return P3X_OK;
@@ -404,6 +432,9 @@ extern "C" int p3x_server_stats_get(p3x_server* s, p3x_server_stats* st) {
   st->completed = s->completed;
   st->column_levels = s->colev;
   st->busy_s = s->t_first < 0 ? 0 : (s->t_last - s->t_first) * 1e-6;
+  st->resident_bytes = s->resident_bytes;
+  st->scratch_bytes = s->scratch_total;
+  st->members = (int)s->members.size();
   if (!s->latencies.empty()) {
     std::vector<double> v = s->latencies;
     std::sort(v.begin(), v.end());

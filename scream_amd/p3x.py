@@ -16,7 +16,11 @@ ICE_TABLE = os.path.join(HERE, "data", "p3_lookup_table_1.dat-v4.1.1")
 
 P3X_OK, P3X_ERR_ARG, P3X_ERR_TABLE_FILE, P3X_ERR_TABLE_VERSION = 0, 1, 2, 3
 P3X_ERR_DEVICE, P3X_ERR_TEMPERATURE, P3X_ERR_NO_MEMORY = 4, 5, 6
+P3X_ERR_SUBSTEPS, P3X_ERR_BUDGET, P3X_ERR_CHECK_VALUES = 7, 8, 9
 P3X_FP64, P3X_MIXED = 0, 1
+# array layouts: level-major [k*ld + i] (numpy shape (nlev, ncol)) or the
+# reference's column-major (column, level) views [i*ld + k] (shape (ncol, nlev))
+P3X_LEVEL_MAJOR, P3X_COLUMN_MAJOR = 0, 1
 
 RUNTIME_KEYS = (
     "max_total_ni", "autoconversion_prefactor", "autoconversion_qc_exponent",
@@ -42,7 +46,8 @@ EXPORTS = ("p3x_runtime_defaults", "p3x_launch_defaults", "p3x_tables_load", "p3
            "p3x_batch_create", "p3x_batch_run", "p3x_batch_tune", "p3x_batch_destroy",
            "p3x_server_defaults", "p3x_server_create", "p3x_server_add_member", "p3x_server_submit",
            "p3x_server_poll", "p3x_server_drain", "p3x_server_stats_get", "p3x_server_destroy",
-           "p3x_process_create", "p3x_process_run", "p3x_process_destroy", "p3x_column_budget",
+           "p3x_process_create", "p3x_process_create_layout", "p3x_process_run", "p3x_process_destroy",
+           "p3x_column_budget",
            "p3x_conservation_errors", "p3x_cld_fraction", "p3x_ice_table_generate",
            "p3x_ice_table_write")
 
@@ -95,14 +100,16 @@ class Launch(C.Structure):
                 ("threads_per_block", C.c_int), ("sed_lanes_per_column", C.c_int),
                 ("part12_waves_per_simd", C.c_int), ("stream", C.c_void_p), ("async_", C.c_int),
                 ("col_status", C.c_void_p), ("sed_substeps", C.c_void_p),
-                ("ice_table_idx", C.c_void_p)]
+                ("ice_table_idx", C.c_void_p), ("layout", C.c_int), ("ld_iface", C.c_int),
+                ("check_values", C.c_void_p)]
 
 
 class ColumnSet(C.Structure):
     _fields_ = [("prognostic_state", PrognosticState), ("diagnostic_inputs", DiagnosticInputs),
                 ("diagnostic_outputs", DiagnosticOutputs), ("history_only", HistoryOnly),
                 ("infrastructure", Infrastructure), ("nj", C.c_int), ("nk", C.c_int), ("ld", C.c_int),
-                ("col_status", C.c_void_p), ("sed_substeps", C.c_void_p), ("ice_table_idx", C.c_void_p)]
+                ("col_status", C.c_void_p), ("sed_substeps", C.c_void_p), ("ice_table_idx", C.c_void_p),
+                ("layout", C.c_int), ("ld_iface", C.c_int), ("check_values", C.c_void_p)]
 
 
 class BatchOptions(C.Structure):
@@ -118,7 +125,7 @@ class ServerOptions(C.Structure):
     _fields_ = [("precision", C.c_int), ("slots", C.c_int), ("max_batch", C.c_int),
                 ("scratch_budget_bytes", C.c_double), ("columns_per_block", C.c_int),
                 ("threads_per_block", C.c_int), ("sed_lanes_per_column", C.c_int),
-                ("part12_waves_per_simd", C.c_int)]
+                ("part12_waves_per_simd", C.c_int), ("hbm_budget_bytes", C.c_double)]
 
 
 class Completion(C.Structure):
@@ -129,7 +136,8 @@ class Completion(C.Structure):
 class ServerStats(C.Structure):
     _fields_ = [("submitted", C.c_longlong), ("completed", C.c_longlong), ("p50_us", C.c_double),
                 ("p99_us", C.c_double), ("max_us", C.c_double), ("column_levels", C.c_double),
-                ("busy_s", C.c_double)]
+                ("busy_s", C.c_double), ("resident_bytes", C.c_double), ("scratch_bytes", C.c_double),
+                ("members", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -139,7 +147,7 @@ BUDGET_FIELDS = ("pseudo_density", "T_mid", "u", "v", "qv", "qc", "qi", "qr", "p
 
 
 class BudgetFields(C.Structure):
-    _fields_ = [(k, C.c_void_p) for k in BUDGET_FIELDS]
+    _fields_ = [(k, C.c_void_p) for k in BUDGET_FIELDS] + [("layout", C.c_int), ("ld", C.c_int)]
 
 
 class AtmFields(C.Structure):
@@ -219,6 +227,9 @@ def load() -> C.CDLL:
         lib.p3x_process_create.argtypes = [C.c_void_p, C.POINTER(Runtime), C.c_int, C.c_int, C.c_int,
                                            C.c_int, C.POINTER(C.c_void_p)]
         lib.p3x_process_create.restype = C.c_int
+        lib.p3x_process_create_layout.argtypes = [C.c_void_p, C.POINTER(Runtime), C.c_int, C.c_int, C.c_int,
+                                                  C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p)]
+        lib.p3x_process_create_layout.restype = C.c_int
         lib.p3x_process_run.argtypes = [C.c_void_p, C.POINTER(AtmFields), C.c_double, C.POINTER(Launch)]
         lib.p3x_process_run.restype = C.c_int
         lib.p3x_process_destroy.argtypes = [C.c_void_p]
@@ -323,9 +334,11 @@ class Context:
              columns_per_block: int = 0, threads_per_block: int = 0,
              sed_lanes_per_column: int = 0, part12_waves_per_simd: int = 0, stream=None,
              async_: bool = False, debug: dict | None = None, exchange_zeroed=False,
-             check: bool = True) -> float:
-        """One p3x_main call; ``f`` maps every field name to a device array.
-        Returns the step's device time in microseconds (0 when async)."""
+             check: bool = True, layout: int = P3X_LEVEL_MAJOR, ld_iface: int = 0) -> float:
+        """One p3x_main call; ``f`` maps every field name to a device array in
+        ``layout`` (level-major, shape (nlev, ncol), or column-major, shape (ncol,
+        nlev), the reference's view order).  Returns the step's device time in
+        microseconds (0 when async)."""
         rt = runtime if runtime is not None else runtime_defaults()
         ps = PrognosticState(*[_addr(f[k]) for k in STATE])
         di = DiagnosticInputs(*[_addr(f[k]) for k in INPUTS])
@@ -342,7 +355,8 @@ class Context:
                     part12_waves_per_simd=int(part12_waves_per_simd), stream=stream,
                     async_=int(bool(async_)), col_status=_addr(dbg.get("col_status")),
                     sed_substeps=_addr(dbg.get("sed_substeps")),
-                    ice_table_idx=_addr(dbg.get("ice_table_idx")))
+                    ice_table_idx=_addr(dbg.get("ice_table_idx")), layout=int(layout),
+                    ld_iface=int(ld_iface), check_values=_addr(dbg.get("check_values")))
         us = C.c_double(0.0)
         st = self._lib.p3x_main(self.handle, C.byref(rt), C.byref(ps), C.byref(di), C.byref(do),
                                 C.byref(inf), C.byref(hi), int(nj), int(nk), C.byref(lo),
@@ -380,8 +394,10 @@ def batch_defaults(**overrides) -> BatchOptions:
 
 
 def column_set(f: dict, nj: int, nk: int, dt: float, predict_nc=True, prescribed_ccn=False,
-               exchange_zeroed=False, ld: int = 0, debug: dict | None = None) -> ColumnSet:
-    """The descriptor of one column set of a batch; ``f`` maps every field name to a device array."""
+               exchange_zeroed=False, ld: int = 0, debug: dict | None = None,
+               layout: int = P3X_LEVEL_MAJOR, ld_iface: int = 0) -> ColumnSet:
+    """The descriptor of one column set of a batch; ``f`` maps every field name to a device array
+    in ``layout``."""
     dbg = debug or {}
     inf = Infrastructure(dt=float(dt), it=1, its=0, ite=nj - 1, kts=0, kte=nk - 1,
                          predictNc=int(bool(predict_nc)), prescribedCCN=int(bool(prescribed_ccn)),
@@ -391,7 +407,8 @@ def column_set(f: dict, nj: int, nk: int, dt: float, predict_nc=True, prescribed
                      DiagnosticOutputs(*[_addr(f[k]) for k in OUTS]),
                      HistoryOnly(*[_addr(f[k]) for k in HIST]), inf, int(nj), int(nk), int(ld),
                      _addr(dbg.get("col_status")), _addr(dbg.get("sed_substeps")),
-                     _addr(dbg.get("ice_table_idx")))
+                     _addr(dbg.get("ice_table_idx")), int(layout), int(ld_iface),
+                     _addr(dbg.get("check_values")))
 
 
 class Batch:
@@ -514,15 +531,18 @@ class P3Microphysics:
     name = "p3"
 
     def __init__(self, ctx: Context, ncol: int, nlev: int, predict_nc=True, prescribed_ccn=False,
-                 runtime: Runtime | None = None):
+                 runtime: Runtime | None = None, layout: int = P3X_LEVEL_MAJOR, ld: int = 0,
+                 ld_iface: int = 0):
         self._lib = load()
         self.ctx, self.ncol, self.nlev = ctx, ncol, nlev
+        self.layout = layout
         rt = runtime if runtime is not None else runtime_defaults()
         h = C.c_void_p()
-        st = self._lib.p3x_process_create(ctx.handle, C.byref(rt), int(ncol), int(nlev),
-                                          int(bool(predict_nc)), int(bool(prescribed_ccn)), C.byref(h))
+        st = self._lib.p3x_process_create_layout(ctx.handle, C.byref(rt), int(ncol), int(nlev),
+                                                 int(bool(predict_nc)), int(bool(prescribed_ccn)),
+                                                 int(layout), int(ld), int(ld_iface), C.byref(h))
         if st != P3X_OK:
-            raise P3Error(st, "p3x_process_create")
+            raise P3Error(st, "p3x_process_create_layout")
         self.handle = h
 
     def run(self, fields: dict, dt: float, precision: int = P3X_FP64, stream=None,
@@ -593,20 +613,20 @@ class MicrophysicsStep:
 
     def __init__(self, ctx: Context, ncol: int, nlev: int, predict_nc=True, prescribed_ccn=False,
                  runtime: Runtime | None = None, ice_cloud_threshold: float = 1e-12,
-                 ice_cloud_for_analysis_threshold: float = 1e-5):
+                 ice_cloud_for_analysis_threshold: float = 1e-5, layout: int = P3X_LEVEL_MAJOR):
         import torch
-        self.ctx, self.ncol, self.nlev = ctx, ncol, nlev
+        self.ctx, self.ncol, self.nlev, self.layout = ctx, ncol, nlev, layout
         self.cld = CldFraction(ctx, ncol, nlev, ice_cloud_threshold, ice_cloud_for_analysis_threshold)
-        self.p3 = P3Microphysics(ctx, ncol, nlev, predict_nc, prescribed_ccn, runtime)
+        self.p3 = P3Microphysics(ctx, ncol, nlev, predict_nc, prescribed_ccn, runtime, layout=layout)
         dev = torch.device("cuda", torch.cuda.current_device())
         self._b = torch.zeros((6, ncol), dtype=torch.float64, device=dev)  # m0 e0 m1 e1 merr eerr
 
     def run(self, fields: dict, dt: float, precision: int = P3X_FP64, stream=None, async_: bool = False):
         m0, e0, m1, e1, me, ee = self._b
         self.cld.run(fields, stream=stream)
-        column_budget(self.ctx, fields, self.ncol, self.nlev, m0, e0, stream=stream)
+        column_budget(self.ctx, fields, self.ncol, self.nlev, m0, e0, stream=stream, layout=self.layout)
         self.p3.run(fields, dt, precision=precision, stream=stream, async_=True)
-        column_budget(self.ctx, fields, self.ncol, self.nlev, m1, e1, stream=stream)
+        column_budget(self.ctx, fields, self.ncol, self.nlev, m1, e1, stream=stream, layout=self.layout)
         conservation_errors(self.ctx, m0, e0, m1, e1, fields["vapor_flux"], fields["water_flux"],
                             fields["ice_flux"], fields["heat_flux"], self.ncol, dt, me, ee, stream=stream)
         if not async_:
@@ -619,10 +639,11 @@ class MicrophysicsStep:
         self.p3.close()
 
 
-def column_budget(ctx: Context, fields: dict, ncol: int, nlev: int, mass, energy, stream=None):
+def column_budget(ctx: Context, fields: dict, ncol: int, nlev: int, mass, energy, stream=None,
+                  layout: int = P3X_LEVEL_MAJOR, ld: int = 0):
     """Enqueue p3x_column_budget; fields maps BUDGET_FIELDS names to device arrays
-    (u, v, ps, phis optional)."""
-    bf = BudgetFields(*[_addr(fields.get(k)) for k in BUDGET_FIELDS])
+    (u, v, ps, phis optional) in ``layout``."""
+    bf = BudgetFields(*[_addr(fields.get(k)) for k in BUDGET_FIELDS], int(layout), int(ld))
     st = load().p3x_column_budget(ctx.handle, C.byref(bf), int(ncol), int(nlev), _addr(mass), _addr(energy),
                                   stream)
     if st != P3X_OK:

@@ -178,3 +178,52 @@ def test_reserve_then_smaller_sets(ctx, oracle_tables):
         got, _, _ = run_gpu(ctx, inp, 300.0, True, False)
         errs = field_errors(got, want)
         assert max(errs.values()) <= RTOL_FP64, errs
+
+
+def _check_values_expected(inp, out, dbg):
+    """check_values' flags per column from a step's outputs: bit 0 T = th exner outside
+    (173, 323) K, bit 1 qv outside (0, 40e-3), on the fully stepped columns only
+    (components/eamxx/src/physics/p3/impl/p3_check_values_impl.hpp:35-60)."""
+    T = out["th"] * (1 / inp["inv_exner"])
+    qv = out["qv"]
+    f = (~((T > 173.0) & (T < 323.0))).astype(np.int32) | 2 * (~((qv > 0.0) & (qv < 40.0e-3))).astype(np.int32)
+    return np.where(dbg["col_status"] == 2, np.bitwise_or.reduce(f, axis=0), 0).astype(np.int32)
+
+
+def test_check_values_flags_and_trap(ctx, oracle_tables):
+    """The opt-in check_values: qv out of bounds is a flag only (status OK); a
+    temperature out of bounds is the trap (P3X_ERR_CHECK_VALUES); the flags equal the
+    bounds applied to the oracle's outputs, and without the option nothing is checked."""
+    import torch
+    from p3_compare import device_fields
+    inp = make_columns(128, 72, seed=93, dry_every=7, evap_every=9)
+    inp["qv"] = inp["qv"].copy()
+    inp["qv"][60, 10] = 0.05
+    want, wdbg, _ = oracle.run_oracle(inp, oracle_tables, 300.0, True, False, math=MATH)
+    d, dbg = device_fields(inp)
+    dbg["check_values"] = torch.full((128,), -7, dtype=torch.int32, device="cuda:0")
+    print("check_values: qv flag", flush=True)
+    ctx.main(d, 128, 72, 300.0, True, False, debug=dbg, stream=torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    exp = _check_values_expected(inp, want, wdbg)
+    assert exp[10] & 2 and not (exp & 1).any()
+    np.testing.assert_array_equal(dbg["check_values"].cpu().numpy(), exp)
+
+    hot = make_columns(128, 72, seed=94, dry_every=7, evap_every=9)
+    hot["th"] = hot["th"].copy()
+    hot["th"][70, 3] = 330.0 * hot["inv_exner"][70, 3]
+    want, wdbg, _ = oracle.run_oracle(hot, oracle_tables, 300.0, True, False, math=MATH)
+    d, dbg = device_fields(hot)
+    dbg["check_values"] = torch.full((128,), -7, dtype=torch.int32, device="cuda:0")
+    print("check_values: T trap", flush=True)
+    with pytest.raises(p3x.P3Error) as e:
+        ctx.main(d, 128, 72, 300.0, True, False, debug=dbg, stream=torch.cuda.current_stream().cuda_stream)
+    assert e.value.status == p3x.P3X_ERR_CHECK_VALUES
+    exp = _check_values_expected(hot, want, wdbg)
+    assert exp[3] & 1
+    np.testing.assert_array_equal(dbg["check_values"].cpu().numpy(), exp)
+    # the step itself is the plain one, and without the option the same step is OK
+    errs = field_errors({k: v.cpu().numpy() for k, v in d.items()}, want)
+    assert max(errs.values()) <= RTOL_FP64, errs
+    got, _, _ = run_gpu(ctx, hot, 300.0, True, False)
+    assert max(field_errors(got, want).values()) <= RTOL_FP64

@@ -60,6 +60,28 @@ def test_exner_reference_pressure_and_dry_dz():
     np.testing.assert_allclose(a["dz"], RD / G * f["pseudo_density"] * f["T_mid"] / 1e5, rtol=1e-15)
 
 
+def test_wet_dz_uses_virtual_temperature():
+    """With moisture, dz is the hypsometric thickness of the virtual temperature
+    T_v = T (1 + (Mdry/Mw - 1) qv), the textbook form with the constants' molecular
+    weights (18.016 and 28.966): the ratio dz(qv) / dz(0) - 1 is exactly that
+    coefficient times qv, so a wrong coefficient or the (1 + qv/eps)/(1 + qv) form
+    (2e-4 away at qv = 0.02) fails."""
+    lib = oracle.load_oracle()
+    f = make_atm_fields(16, 72, seed=7)
+    rng = np.random.default_rng(7)
+    f["qv"] = rng.uniform(1e-4, 3e-2, f["qv"].shape)
+    a = _atm(f)
+    lib.p3o_preamble(C.byref(oracle._atm_struct(a)), 16, 72)
+    c1 = 28.966 / 18.016 - 1.0
+    want = RD / G * f["pseudo_density"] * (f["T_mid"] * (1.0 + c1 * f["qv"])) / f["p_mid"]
+    np.testing.assert_allclose(a["dz"], want, rtol=2e-15, atol=0)
+    f0 = dict(f)
+    f0["qv"] = np.zeros_like(f["qv"])
+    a0 = _atm(f0)
+    lib.p3o_preamble(C.byref(oracle._atm_struct(a0)), 16, 72)
+    np.testing.assert_allclose(a["dz"] / a0["dz"] - 1.0, c1 * f["qv"], rtol=1e-11, atol=0)
+
+
 def test_rain_fraction_maximum_overlap_brute_force():
     lib = oracle.load_oracle()
     f = make_atm_fields(16, 72, seed=5)
