@@ -60,8 +60,10 @@ enum {
   P3X_ERR_NO_MEMORY = 6,    /* device allocation failed                            */
   P3X_ERR_SUBSTEPS = 7,     /* a sedimentation loop hit the 100000-substep cap     */
   P3X_ERR_BUDGET = 8,       /* served form: a member does not fit the HBM budget   */
-  P3X_ERR_CHECK_VALUES = 9  /* with p3x_launch.check_values: a level's T left     */
+  P3X_ERR_CHECK_VALUES = 9, /* with p3x_launch.check_values: a level's T left     */
                             /* (173, 323) K after the step (p3_check_values_impl.hpp:25-79) */
+  P3X_ERR_SCRATCH_LIMIT = 10 /* p3x_main_noalloc: the column set needs more scratch */
+                            /* than its fixed array holds (P3X_NOALLOC_SCRATCH_BYTES) */
 };
 
 /* ---- P3Runtime (p3_functions.hpp:112-158); defaults from p3x_runtime_defaults ---- */
@@ -218,6 +220,27 @@ int p3x_main(p3x_context* ctx, const p3x_runtime* runtime,
              const p3x_infrastructure* infrastructure,
              const p3x_history_only* history_only,
              int nj, int nk, const p3x_launch* opts, double* elapsed_us);
+
+/* No counterpart in the reference: p3x_main for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Same arguments, kernels and results as p3x_main;
+ * the scratch of the step (7 x nk x nj doubles + nj ints, rounded up to 256
+ * bytes) comes from a fixed device array of P3X_NOALLOC_SCRATCH_BYTES that the
+ * library holds on every GPU it runs on until the process ends, instead of
+ * from the context, and the call allocates nothing (no hipMalloc, hipFree or
+ * allocating library routine).  A column set needing more returns
+ * P3X_ERR_SCRATCH_LIMIT without launching (21600 x 72 and 16384 x 128 fit).
+ * Every p3x_main_noalloc call on one GPU, from any context or stream, shares
+ * that array, which keeps the last call's contents: calls that overlap on one
+ * GPU race (order them on one stream, or with events). */
+#define P3X_NOALLOC_SCRATCH_BYTES 134217728ull /* 128 MiB */
+int p3x_main_noalloc(p3x_context* ctx, const p3x_runtime* runtime,
+                     const p3x_prognostic_state* prognostic_state,
+                     const p3x_diagnostic_inputs* diagnostic_inputs,
+                     const p3x_diagnostic_outputs* diagnostic_outputs,
+                     const p3x_infrastructure* infrastructure,
+                     const p3x_history_only* history_only,
+                     int nj, int nk, const p3x_launch* opts, double* elapsed_us);
 
 /* Wait for the context's enqueued work and return its status. */
 int p3x_sync(p3x_context* ctx);

@@ -11,10 +11,43 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "../../include/p3x.h"
 #include "p3x_internal.hpp"
+
+// The scratch of p3x_main_noalloc: one fixed array per GPU, allocated with the
+// code object and held until the process ends.  In host code this name is only
+// the runtime's handle for the symbol, not a device address: p3x_init resolves
+// the array's address on its device (hipGetSymbolAddress) and the calls pass
+// that address to the kernels.
+static __device__ __align__(256) unsigned char scratch[P3X_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+// Device address of `scratch` per device ordinal, resolved once per device.
+std::mutex g_noalloc_mu;
+std::vector<double*> g_noalloc_ptr;
+
+// With `dev` current: the address of `scratch` on it (resolved on first use).
+hipError_t noalloc_scratch_on(int dev, double** out) {
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (dev < 0) return hipErrorInvalidDevice;
+  if ((size_t)dev < g_noalloc_ptr.size() && g_noalloc_ptr[dev]) {
+    *out = g_noalloc_ptr[dev];
+    return hipSuccess;
+  }
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  if (!ptr) return hipErrorInvalidSymbol;
+  if (g_noalloc_ptr.size() <= (size_t)dev) g_noalloc_ptr.resize(dev + 1, nullptr);
+  g_noalloc_ptr[dev] = (double*)ptr;
+  *out = (double*)ptr;
+  return hipSuccess;
+}
+}  // namespace
+
 
 
 namespace p3x_detail {
@@ -251,6 +284,9 @@ extern "C" const char* p3x_strerror(int st) {
     case P3X_ERR_SUBSTEPS: return "sedimentation needed more than 100000 CFL substeps (corrupt input)";
     case P3X_ERR_BUDGET: return "the member does not fit the server's HBM budget";
     case P3X_ERR_CHECK_VALUES: return "check_values: a temperature outside (173, 323) K after the step";
+    case P3X_ERR_SCRATCH_LIMIT:
+      return "p3x_main_noalloc: the column set needs more scratch than the fixed array holds "
+             "(P3X_NOALLOC_SCRATCH_BYTES)";
     default: return "unknown P3 status";
   }
 }
@@ -276,6 +312,12 @@ extern "C" int p3x_init(const p3x_table_data* t, p3x_context** out) {
   if (e != hipSuccess) {
     p3x_finalize(c);
     return dev_fail(e);
+  }
+  // p3x_main_noalloc's array on this device (the device is current here); a
+  // failure only disables that entry point (it then returns P3X_ERR_DEVICE)
+  if (noalloc_scratch_on(c->device, &c->noalloc_scratch) != hipSuccess) {
+    (void)hipGetLastError();
+    c->noalloc_scratch = nullptr;
   }
   *out = c;
   return P3X_OK;
@@ -337,10 +379,21 @@ extern "C" int p3x_main(p3x_context* c, const p3x_runtime* runtime, const p3x_pr
   return main_impl(c, runtime, ps, di, dout, infra, hist, nj, nk, opts, elapsed_us, nullptr);
 }
 
+extern "C" int p3x_main_noalloc(p3x_context* c, const p3x_runtime* runtime, const p3x_prognostic_state* ps,
+                                const p3x_diagnostic_inputs* di, const p3x_diagnostic_outputs* dout,
+                                const p3x_infrastructure* infra, const p3x_history_only* hist, int nj, int nk,
+                                const p3x_launch* opts, double* elapsed_us) {
+  if (!c) return P3X_ERR_ARG;
+  if (!c->noalloc_scratch) return P3X_ERR_DEVICE;
+  return main_impl(c, runtime, ps, di, dout, infra, hist, nj, nk, opts, elapsed_us, nullptr, c->noalloc_scratch,
+                   (size_t)P3X_NOALLOC_SCRATCH_BYTES);
+}
+
 int p3x_detail::main_impl(p3x_context* c, const p3x_runtime* runtime, const p3x_prognostic_state* ps,
                           const p3x_diagnostic_inputs* di, const p3x_diagnostic_outputs* dout,
                           const p3x_infrastructure* infra, const p3x_history_only* hist, int nj, int nk,
-                          const p3x_launch* opts, double* elapsed_us, const p3d::Amble* amble) {
+                          const p3x_launch* opts, double* elapsed_us, const p3d::Amble* amble,
+                          double* fixed_scratch, size_t fixed_bytes) {
   if (!c || !runtime || !ps || !di || !dout || !infra || !hist) return P3X_ERR_ARG;
   if (nj < 0 || nk <= 0 || !(infra->dt > 0)) return P3X_ERR_ARG;
   p3x_launch o;
@@ -362,15 +415,19 @@ int p3x_detail::main_impl(p3x_context* c, const p3x_runtime* runtime, const p3x_
   if (nj == 0) return P3X_OK;
   if (!fields_present(ps, di, dout, hist)) return P3X_ERR_ARG;
 
-  {
+  double* scr = fixed_scratch;
+  if (scr) {
+    if (scratch_bytes_for(nj, nk) > fixed_bytes) return P3X_ERR_SCRATCH_LIMIT;
+  } else {
     const int st = grow_scratch(c, scratch_bytes_for(nj, nk));
     if (st != P3X_OK) return st;
+    scr = c->d_scratch;
   }
 
   p3d::KParams k;
   {
     const int st = fill_params(c, runtime, ps, di, dout, infra, hist, nj, nk, o.layout, o.ld, o.ld_iface,
-                               o.col_status, o.sed_substeps, o.ice_table_idx, c->d_scratch, &k);
+                               o.col_status, o.sed_substeps, o.ice_table_idx, scr, &k);
     if (st != P3X_OK) return st;
   }
   k.dbg_check = o.check_values;

@@ -26,6 +26,9 @@ struct p3x_context {
   hipStream_t main_stream = nullptr;
   int main_launched = 0;
   int pending_timing = 0;
+  // device address of p3x_main_noalloc's fixed scratch array on this context's
+  // device (NULL if it could not be resolved at p3x_init)
+  double* noalloc_scratch = nullptr;
 };
 
 namespace p3x_detail {
@@ -41,11 +44,14 @@ int grow_scratch(p3x_context* c, size_t need);
 int set_geometry(int nj, int nk, int layout, int ld, int ld_iface, p3d::KParams* k);
 // Elements a level field of k spans from its first element to its last.
 size_t level_span(const p3d::KParams& k);
-// p3x_main with (amble != NULL) the P3Microphysics pre/post-amble fused into its kernels.
+// p3x_main with (amble != NULL) the P3Microphysics pre/post-amble fused into its
+// kernels; fixed_scratch != NULL: the step's scratch is that array of
+// fixed_bytes bytes (the context's scratch is not touched or grown).
 int main_impl(p3x_context* c, const p3x_runtime* runtime, const p3x_prognostic_state* ps,
               const p3x_diagnostic_inputs* di, const p3x_diagnostic_outputs* dout,
               const p3x_infrastructure* infra, const p3x_history_only* hist, int nj, int nk,
-              const p3x_launch* opts, double* elapsed_us, const p3d::Amble* amble);
+              const p3x_launch* opts, double* elapsed_us, const p3d::Amble* amble,
+              double* fixed_scratch = nullptr, size_t fixed_bytes = 0);
 // KParams of one column set; scratch (NULL: set later) points at SC_COUNT*nj*nk
 // doubles followed by nj ints.  P3X_ERR_ARG for an invalid layout.
 int fill_params(const p3x_context* c, const p3x_runtime* runtime, const p3x_prognostic_state* ps,

@@ -17,6 +17,9 @@ ICE_TABLE = os.path.join(HERE, "data", "p3_lookup_table_1.dat-v4.1.1")
 P3X_OK, P3X_ERR_ARG, P3X_ERR_TABLE_FILE, P3X_ERR_TABLE_VERSION = 0, 1, 2, 3
 P3X_ERR_DEVICE, P3X_ERR_TEMPERATURE, P3X_ERR_NO_MEMORY = 4, 5, 6
 P3X_ERR_SUBSTEPS, P3X_ERR_BUDGET, P3X_ERR_CHECK_VALUES = 7, 8, 9
+P3X_ERR_SCRATCH_LIMIT = 10
+# the fixed scratch array of p3x_main_noalloc, per GPU (bytes)
+P3X_NOALLOC_SCRATCH_BYTES = 128 << 20
 P3X_FP64, P3X_MIXED = 0, 1
 # array layouts: level-major [k*ld + i] (numpy shape (nlev, ncol)) or the
 # reference's column-major (column, level) views [i*ld + k] (shape (ncol, nlev))
@@ -41,7 +44,7 @@ ALL_FIELDS = STATE + INPUTS + OUTS + HIST
 
 # Every symbol include/p3x.h declares.
 EXPORTS = ("p3x_runtime_defaults", "p3x_launch_defaults", "p3x_tables_load", "p3x_init", "p3x_reserve",
-           "p3x_finalize", "p3x_main", "p3x_sync", "p3x_last_kernel_times", "p3x_strerror",
+           "p3x_finalize", "p3x_main", "p3x_main_noalloc", "p3x_sync", "p3x_last_kernel_times", "p3x_strerror",
            "p3x_batch_defaults",
            "p3x_batch_create", "p3x_batch_run", "p3x_batch_tune", "p3x_batch_destroy",
            "p3x_server_defaults", "p3x_server_create", "p3x_server_add_member", "p3x_server_submit",
@@ -193,6 +196,12 @@ def load() -> C.CDLL:
                                  C.POINTER(Infrastructure), C.POINTER(HistoryOnly), C.c_int,
                                  C.c_int, C.POINTER(Launch), C.POINTER(C.c_double)]
         lib.p3x_main.restype = C.c_int
+        if hasattr(lib, "p3x_main_noalloc"):  # (every export is checked by the ABI test)
+            lib.p3x_main_noalloc.argtypes = [C.c_void_p, C.POINTER(Runtime), C.POINTER(PrognosticState),
+                                     C.POINTER(DiagnosticInputs), C.POINTER(DiagnosticOutputs),
+                                     C.POINTER(Infrastructure), C.POINTER(HistoryOnly), C.c_int,
+                                     C.c_int, C.POINTER(Launch), C.POINTER(C.c_double)]
+            lib.p3x_main_noalloc.restype = C.c_int
         lib.p3x_sync.argtypes = [C.c_void_p]
         lib.p3x_sync.restype = C.c_int
         lib.p3x_last_kernel_times.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double)]
@@ -334,11 +343,13 @@ class Context:
              columns_per_block: int = 0, threads_per_block: int = 0,
              sed_lanes_per_column: int = 0, part12_waves_per_simd: int = 0, stream=None,
              async_: bool = False, debug: dict | None = None, exchange_zeroed=False,
-             check: bool = True, layout: int = P3X_LEVEL_MAJOR, ld_iface: int = 0) -> float:
+             check: bool = True, layout: int = P3X_LEVEL_MAJOR, ld_iface: int = 0,
+             noalloc: bool = False) -> float:
         """One p3x_main call; ``f`` maps every field name to a device array in
         ``layout`` (level-major, shape (nlev, ncol), or column-major, shape (ncol,
-        nlev), the reference's view order).  Returns the step's device time in
-        microseconds (0 when async)."""
+        nlev), the reference's view order).  ``noalloc``: call p3x_main_noalloc
+        (the step's scratch from the library's fixed per-GPU array).  Returns the
+        step's device time in microseconds (0 when async)."""
         rt = runtime if runtime is not None else runtime_defaults()
         ps = PrognosticState(*[_addr(f[k]) for k in STATE])
         di = DiagnosticInputs(*[_addr(f[k]) for k in INPUTS])
@@ -358,11 +369,11 @@ class Context:
                     ice_table_idx=_addr(dbg.get("ice_table_idx")), layout=int(layout),
                     ld_iface=int(ld_iface), check_values=_addr(dbg.get("check_values")))
         us = C.c_double(0.0)
-        st = self._lib.p3x_main(self.handle, C.byref(rt), C.byref(ps), C.byref(di), C.byref(do),
-                                C.byref(inf), C.byref(hi), int(nj), int(nk), C.byref(lo),
-                                C.byref(us))
+        fn = self._lib.p3x_main_noalloc if noalloc else self._lib.p3x_main
+        st = fn(self.handle, C.byref(rt), C.byref(ps), C.byref(di), C.byref(do),
+                C.byref(inf), C.byref(hi), int(nj), int(nk), C.byref(lo), C.byref(us))
         if check and st != P3X_OK:
-            raise P3Error(st, "p3x_main")
+            raise P3Error(st, "p3x_main_noalloc" if noalloc else "p3x_main")
         return us.value if st == P3X_OK else float("nan")
 
     def sync(self) -> int:
