@@ -41,6 +41,10 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-forms", action="store_true", help="skip the 16384x128 forms")
+    ap.add_argument("--no-forms-batch", action="store_true",
+                    help="skip the mixed batched leg of the 16384x128 forms")
+    ap.add_argument("--no-column-major", action="store_true",
+                    help="skip the headline step on column-major (column, level) arrays")
     ap.add_argument("--no-process", action="store_true",
                     help="skip the P3Microphysics process (run_impl) leg")
     ap.add_argument("--batch-sets", type=int, default=64,
@@ -308,6 +312,25 @@ def run_batched(ctx, sb, precision, steps, tune_reps=2):
             "state": "restored before every step (one device copy, outside the events)"}
 
 
+def forms_batched(ctx, ncol, nlev, dt, forms, steps):
+    """The mixed form batched on the 16384 x 128 workload: the same columns as 8 sets of
+    2048 (one graph replay per step, autotuned), against the plain FP64 drop-in on the
+    whole set (forms["fp64"])."""
+    import numpy as np
+    from scream_amd import p3x
+    from scream_amd.inputs import make_columns
+    whole = make_columns(ncol, nlev, seed=2)
+    parts = [{k: np.ascontiguousarray(v[..., j * (ncol // 8):(j + 1) * (ncol // 8)])
+              for k, v in whole.items()} for j in range(8)]
+    sb = SetBatch([(ncol // 8, nlev, dt)] * 8, 0, inputs=parts)
+    del whole, parts
+    mb = run_batched(ctx, sb, p3x.P3X_MIXED, steps)
+    del sb
+    mb["workload"] = f"mixed batched form, 8 column sets of {ncol // 8} x {nlev}, dt={dt:g} s"
+    mb["over_fp64_drop_in"] = forms["fp64"]["kernel_us_per_step"] / (1000.0 * mb["kernel_ms_per_step"])
+    return mb
+
+
 def batched(ctx, nsets, steps, seed=4):
     """Batched form: nsets independent column sets, ncol log-uniform in [256, 8192],
     nlev from {72, 128}, dt from {100, 300, 1800} s, mixed precision, one graph replay
@@ -538,14 +561,16 @@ def summarize_pmc(fetch_dir, write_dir, out_path=None):
     scale_f = COPY_PROBE_BYTES / copy_f  # bytes per counter KiB, reads
     scale_w = COPY_PROBE_BYTES / copy_w  # bytes per counter KiB, writes
     kern = {}
-    # part 1 is p3_part1_lvl_kernel (one thread per level and column) or p3_part1_kernel
+    # part 1 is p3_part1_lvl_kernel (one thread per level and column) or p3_part1_kernel;
+    # sedimentation is two kernels (the ice group and the liquid group), summed
     for key, name in (("p3_part1", "p3_part1_kernel"), ("p3_part2", "p3_part2_kernel"),
                       ("p3_sed", "p3_sed_kernel"), ("p3_part3", "p3_part3_kernel")):
         f = pick(fetch, key)
         w = pick(write, key)
         if f and w:
-            kern[name] = {"FETCH_SIZE_KiB": max(f.values()), "WRITE_SIZE_KiB": max(w.values()),
-                          "bytes": max(f.values()) * scale_f + max(w.values()) * scale_w}
+            kern[name] = {"FETCH_SIZE_KiB": sum(f.values()), "WRITE_SIZE_KiB": sum(w.values()),
+                          "bytes": sum(f.values()) * scale_f + sum(w.values()) * scale_w,
+                          "kernels": sorted(f)}
     out = {"g64_fp64": {
         "workload": "p3_main FP64 drop-in, 21600x72, dt=1800 s (bench.py --no-cpu --no-forms "
                     "--no-process --batch-sets 0 --serve-seconds 0)",
@@ -622,16 +647,17 @@ def main():
         # algorithmic bytes per step over its event-timed duration, and its measured HBM bytes
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "p3_sed_kernel + p3_part3_lvl_kernel"},
+                     "kernel": "p3_sed_kernel (ice group, liquid group) + p3_part3_lvl_kernel"},
     }
     del w
-    # the same step on the reference's own (column, level) layout of the fields
-    wc = Workload(ncol, nlev, seed=1, layout=p3x.P3X_COLUMN_MAJOR)
-    ms_c, kern_c = time_steps(ctx, wc, dt, p3x.P3X_FP64, args.steps, args.warmup)
-    line["column_major"] = {"workload": workload + ", column-major (column, level) arrays",
-                            "column_levels_per_s": colev / (ms_c / 1000.0), "ms_per_step": ms_c,
-                            "kernel_us_per_step": kern_c, "kernel_time_over_level_major": kern_c / kern_us}
-    del wc
+    if not args.no_column_major:
+        # the same step on the reference's own (column, level) layout of the fields
+        wc = Workload(ncol, nlev, seed=1, layout=p3x.P3X_COLUMN_MAJOR)
+        ms_c, kern_c = time_steps(ctx, wc, dt, p3x.P3X_FP64, args.steps, args.warmup)
+        line["column_major"] = {"workload": workload + ", column-major (column, level) arrays",
+                                "column_levels_per_s": colev / (ms_c / 1000.0), "ms_per_step": ms_c,
+                                "kernel_us_per_step": kern_c, "kernel_time_over_level_major": kern_c / kern_us}
+        del wc
     if not args.no_process:
         line["process"] = process_step(ctx, ncol, nlev, dt, args.steps, args.warmup)
         pc = process_step(ctx, ncol, nlev, dt, args.steps, args.warmup, layout=p3x.P3X_COLUMN_MAJOR)
@@ -650,17 +676,8 @@ def main():
         forms["mixed_over_fp64_kernel"] = forms["fp64"]["kernel_us_per_step"] / forms["mixed"]["kernel_us_per_step"]
         # the mixed form batched: the same 16384 x 128 columns as 8 sets of 2048 (one graph
         # replay per step, autotuned), against the plain FP64 drop-in on the whole set
-        from scream_amd.inputs import make_columns
-        whole = make_columns(ncol2, nlev2, seed=2)
-        parts = [{k: np.ascontiguousarray(v[..., j * (ncol2 // 8):(j + 1) * (ncol2 // 8)])
-                  for k, v in whole.items()} for j in range(8)]
-        sb = SetBatch([(ncol2 // 8, nlev2, dt2)] * 8, 0, inputs=parts)
-        del whole, parts
-        mb = run_batched(ctx, sb, p3x.P3X_MIXED, max(5, args.steps // 2))
-        del sb
-        mb["workload"] = f"mixed batched form, 8 column sets of {ncol2 // 8} x {nlev2}, dt={dt2:g} s"
-        mb["over_fp64_drop_in"] = forms["fp64"]["kernel_us_per_step"] / (1000.0 * mb["kernel_ms_per_step"])
-        forms["mixed_batched"] = mb
+        if not args.no_forms_batch:
+            forms["mixed_batched"] = forms_batched(ctx, ncol2, nlev2, dt2, forms, max(5, args.steps // 2))
         errs = mixed_error(ctx, w2, dt2)
         forms["mixed_error_vs_fp64"] = {
             "worst_field": max(errs, key=lambda k: errs[k]["max_rel_to_field_max"]),

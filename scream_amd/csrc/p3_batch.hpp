@@ -88,14 +88,27 @@ __global__ __launch_bounds__(kTileThreads) void p3_part3_lvl_batch(const KParams
   }
 }
 
-template <class R, int LPC, int L, int WPB>
-__global__ __launch_bounds__(64 * WPB, sed_waves_per_simd<L>()) void p3_sed_batch(const KParams* __restrict__ sets,
+// Sedimentation of species group G (SED_LIQ or SED_ICE).  The sets of a batch
+// share their context's tables, so the ice group stages the first set's ice
+// records in LDS once per persistent block.
+template <class R, int LPC, int L, int WPB, int G>
+__global__ __launch_bounds__(64 * WPB, (sed_waves_per_simd<R, G, L>())) void p3_sed_batch(const KParams* __restrict__ sets,
                                                          const int* __restrict__ first, int nsets) {
   const auto sets4 = (const __attribute__((address_space(4))) KParams*)sets;
   const int total = first[nsets];
-  for (int vb = blockIdx.x; vb < total; vb += gridDim.x) {
-    const int s = find_set(first, nsets, vb);
-    sed_block<R, LPC, L, WPB>(*(const KParams*)(sets4 + s), vb - first[s]);
+  if constexpr (G == SED_ICE) {
+    __shared__ __align__(16) R s_tab[kIceSedTableLen];
+    if ((int)blockIdx.x >= total) return;  // the whole block leaves before the barrier
+    stage_ice_table<R>(TabOf<R>::get(((const KParams*)sets4)->tab).ice_sed, s_tab);
+    for (int vb = blockIdx.x; vb < total; vb += gridDim.x) {
+      const int s = find_set(first, nsets, vb);
+      sed_block<R, LPC, L, WPB, G>(*(const KParams*)(sets4 + s), vb - first[s], s_tab);
+    }
+  } else {
+    for (int vb = blockIdx.x; vb < total; vb += gridDim.x) {
+      const int s = find_set(first, nsets, vb);
+      sed_block<R, LPC, L, WPB, G>(*(const KParams*)(sets4 + s), vb - first[s], nullptr);
+    }
   }
 }
 

@@ -260,28 +260,53 @@ __global__ __launch_bounds__(NT) void p3_part1_kernel(const KParams p) {
 }
 
 // Tiles of the pointwise kernels (part 1, part 2, part 3): 256 threads, one
-// (level, column) point each.  Level-major: 64 consecutive columns x 4 levels,
-// one level per wavefront, columns fastest, so a wavefront reads 64
-// consecutive doubles of a row.  Column-major: 256 consecutive points of the
-// column-by-column order (levels fastest), so a wavefront reads consecutive
-// levels of one or two columns.
+// (level, column) point each, arranged so that a wavefront's loads coalesce:
+//  * level-major: 64 consecutive columns x 4 levels, one level per wavefront,
+//    columns fastest, so a wavefront reads 64 consecutive doubles of a row
+//    (and holds points of one level, whose work is alike);
+//  * column-major, two arrangements:
+//    - PW_RUNS (part 1, part 3, memory-bound): 256 consecutive points of the
+//      column-by-column order, so a wavefront reads 512 contiguous bytes;
+//    - PW_BLOCKS (part 2): a wavefront holds 8 columns x 8 consecutive levels
+//      (levels fastest, 8 runs of 64 bytes per load), a block 32 columns x 8
+//      levels, so that a wavefront's points are of nearby levels: the upper
+//      levels are mostly dry and skip part 2's processes, and a wavefront
+//      spanning a whole column would run them for its few wet lanes (part 2
+//      330 -> 270 us at 21600 x 72 against PW_RUNS).
+//    Both take the same grid (the larger of their tile counts); tiles past an
+//    arrangement's end hold no points.
 constexpr int kPart2Cols = 64, kPart2Levels = 4, kTileThreads = kPart2Cols * kPart2Levels;
+constexpr int kBlkCols = 32, kBlkLevels = 8;
+enum PwMap { PW_RUNS = 0, PW_BLOCKS = 1 };
 __host__ __device__ __forceinline__ int part2_tiles(int ncol, int nlev) {
   return ((ncol + kPart2Cols - 1) / kPart2Cols) * ((nlev + kPart2Levels - 1) / kPart2Levels);
 }
 __host__ __device__ __forceinline__ int pw_tiles(const KParams& p) {
-  if (p.colmajor) return (int)(((long long)p.ncol * p.nlev + kTileThreads - 1) / kTileThreads);
+  if (p.colmajor) {
+    const int runs = (int)(((long long)p.ncol * p.nlev + kTileThreads - 1) / kTileThreads);
+    const int blocks = ((p.ncol + kBlkCols - 1) / kBlkCols) * ((p.nlev + kBlkLevels - 1) / kBlkLevels);
+    return runs > blocks ? runs : blocks;
+  }
   return part2_tiles(p.ncol, p.nlev);
 }
 // The point of this thread in tile `tile`; false past the end of the set.
 // (Column-major point counts are below 2^31: the host refuses larger sets.)
+template <int MAP>
 __device__ __forceinline__ bool pw_point(const KParams& p, int tile, int& i, int& k) {
   if (p.colmajor) {
-    const unsigned e = (unsigned)tile * kTileThreads + threadIdx.x;
-    const unsigned nl = (unsigned)p.nlev;
-    i = (int)(e / nl);
-    k = (int)(e - (unsigned)i * nl);
-    return i < p.ncol;
+    if constexpr (MAP == PW_RUNS) {
+      const unsigned e = (unsigned)tile * kTileThreads + threadIdx.x;
+      const unsigned nl = (unsigned)p.nlev;
+      i = (int)(e / nl);
+      k = (int)(e - (unsigned)i * nl);
+      return i < p.ncol;
+    } else {
+      const int ntc = (p.ncol + kBlkCols - 1) / kBlkCols;
+      const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+      i = (tile % ntc) * kBlkCols + w * 8 + (l >> 3);
+      k = (tile / ntc) * kBlkLevels + (l & 7);
+      return i < p.ncol && k < p.nlev;
+    }
   }
   const int ntc = (p.ncol + kPart2Cols - 1) / kPart2Cols;
   i = (tile % ntc) * kPart2Cols + (threadIdx.x & 63);
@@ -290,10 +315,13 @@ __device__ __forceinline__ bool pw_point(const KParams& p, int tile, int& i, int
 }
 
 // ORs `bit` into the status word of column i for every lane whose `set` is
-// true.  Level-major lanes hold distinct columns; column-major lanes hold runs
-// of consecutive levels of a column (lane l's run starts min(l, k) lanes
-// below it), so one lane per run and wavefront makes the atomic.  Every lane
-// of the wavefront that is still running must call this.
+// true, one atomic per column and wavefront.  Level-major lanes hold distinct
+// columns.  Column-major lanes hold runs of one column's consecutive levels:
+// under PW_RUNS lane l's run starts min(l, k) lanes below it, under PW_BLOCKS
+// the runs are the aligned groups of 8 lanes; the lowest lane of a run with
+// `set` makes the atomic.  Every lane of the wavefront that is still running
+// must call this.
+template <int MAP>
 __device__ __forceinline__ void colstat_or(const KParams& p, int i, int k, bool set, int bit) {
   if (!p.colmajor) {
     if (set) atomicOr(&p.colstat[i], bit);
@@ -301,8 +329,13 @@ __device__ __forceinline__ void colstat_or(const KParams& p, int i, int k, bool 
   }
   const unsigned long long m = __ballot(set);
   const int l = threadIdx.x & 63;
-  const int below = k < l ? k : l;
-  const unsigned long long run_below = ((1ull << l) - 1) ^ ((1ull << (l - below)) - 1);
+  unsigned long long run_below;
+  if constexpr (MAP == PW_RUNS) {
+    const int below = k < l ? k : l;
+    run_below = ((1ull << l) - 1) ^ ((1ull << (l - below)) - 1);
+  } else {
+    run_below = (0xFFull << (l & ~7)) & ((1ull << l) - 1);
+  }
   if (set && !(m & run_below)) atomicOr(&p.colstat[i], bit);
 }
 
@@ -313,10 +346,10 @@ __device__ __forceinline__ void colstat_or(const KParams& p, int i, int k, bool 
 template <class R>
 __device__ __forceinline__ void part1_tile(const KParams& p, int tile) {
   int i, k;
-  if (!pw_point(p, tile, i, k)) return;
+  if (!pw_point<PW_RUNS>(p, tile, i, k)) return;
   unsigned err = 0;
   const int flags = part1_level<R>(p, k, i, err);
-  colstat_or(p, i, k, (flags & 3) != 0, kStatusPart1);
+  colstat_or<PW_RUNS>(p, i, k, (flags & 3) != 0, kStatusPart1);
   if (k == 0 && p.dbg_check) p.dbg_check[i] = 0;  // part 3 ORs the column's check_values flags
   if (err) atomicOr(p.err, err);
 }
@@ -339,7 +372,7 @@ __global__ __launch_bounds__(256) void p3_colstat_clear_kernel(int* colstat, int
 template <class R>
 __device__ __forceinline__ void part2_tile(const KParams& p, int tile) {
   int i, k;
-  if (!pw_point(p, tile, i, k)) return;
+  if (!pw_point<PW_BLOCKS>(p, tile, i, k)) return;
   const size_t x = p.at(i, k);
   auto scr = [&](int f) -> double { return p.sc(f, i, k); };
   unsigned err = 0;
@@ -403,7 +436,7 @@ __device__ __forceinline__ void part2_tile(const KParams& p, int tile) {
     }
   }
   // hydrometeorsPresent after part 2 (P3I/p3_main_impl.hpp:255-260): a column flag
-  colstat_or(p, i, k, hydro, kStatusPart2);
+  colstat_or<PW_BLOCKS>(p, i, k, hydro, kStatusPart2);
   if (err) atomicOr(p.err, err);
 }
 

@@ -14,12 +14,15 @@ namespace p3d {
 //   part 2: one thread per (level, column) on the pointwise tiles, register
 //     allocation for wpe = 1 or 2 wavefronts per SIMD (2: fewer registers,
 //     with spills in FP64);
-//   sedimentation: lpc lanes per column (16, 32 or 64), each lane
-//     holding L = ceil(nlev / lpc) levels (at most 8), 4 wavefronts per block;
+//   sedimentation, two kernels: the liquid group (cloud, rain) on lpc lanes
+//     per column and the ice group on lpc_ice (16, 32 or 64 each), each lane
+//     holding L = ceil(nlev / lanes) levels (at most 8), 4 wavefronts per
+//     block (the batched form runs both groups on lpc);
 //   part 3: one thread per (level, column) on the pointwise tiles.
 struct Shape {
   int cb, nt, lpc, wpe;
-  int p1lvl;  // 1: part 1 one thread per (level, column); 0: the cb x nt block kernel
+  int p1lvl;    // 1: part 1 one thread per (level, column); 0: the cb x nt block kernel
+  int lpc_ice;  // lanes per column of the ice group's sedimentation kernel
 };
 
 // Largest levels-per-lane count of the sedimentation kernel.

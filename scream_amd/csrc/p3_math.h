@@ -83,11 +83,15 @@ P3M_FN double p3m_exp(double x) {
  * here with the exact series coefficients up to s^22). */
 P3M_FN void p3m_log_dd(double x, double* hi, double* lo) {
   uint64_t u = p3m_bits(x);
-  if (!(x == x) || x < 0) { *hi = p3m_nan(); *lo = 0; return; }
-  if (x == 0) { *hi = -p3m_inf(); *lo = 0; return; }
-  if (u >= 0x7ff0000000000000ULL) { *hi = x; *lo = 0; return; }
   int e = 0;
-  if (u < 0x0010000000000000ULL) { x *= 18014398509481984.0; u = p3m_bits(x); e = -54; }
+  /* One unsigned compare separates the positive normal finite arguments (the
+   * only ones P3 passes) from zero, subnormals, negatives, infinities and NaN. */
+  if (u - 0x0010000000000000ULL >= 0x7fe0000000000000ULL) {
+    if (!(x == x) || x < 0) { *hi = p3m_nan(); *lo = 0; return; }
+    if (x == 0) { *hi = -p3m_inf(); *lo = 0; return; }
+    if (u >= 0x7ff0000000000000ULL) { *hi = x; *lo = 0; return; }
+    x *= 18014398509481984.0; u = p3m_bits(x); e = -54;  /* subnormal */
+  }
   e += (int)(u >> 52) - 1023;
   double m = p3m_from_bits((u & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL);
   if (m > P3M_SQRT2) { m *= 0.5; e += 1; }

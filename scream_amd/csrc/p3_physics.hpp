@@ -390,12 +390,14 @@ __device__ __forceinline__ R calc_bulk_rho_rime(const Runtime& rt, R qi_tot, R& 
   const bool lt_min = rho_rime < R(rt.min_rime_rho), gt_max = rho_rime > R(rt.max_rime_rho);
   if (gt && lt_min) rho_rime = R(rt.min_rime_rho);
   if (gt && gt_max) rho_rime = R(rt.max_rime_rho);
-  if (gt && (gt_max || lt_min)) bi_rim = qi_rim / rho_rime;
+  const bool clamped = gt && (gt_max || lt_min);
   if (lt) { qi_rim = R(0); bi_rim = R(0); rho_rime = R(0); }
-  if (qi_rim > qi_tot && rho_rime > R(0)) {
-    qi_rim = qi_tot;
-    bi_rim = qi_rim / rho_rime;
-  }
+  // The reference sets bi_rim = qi_rim / rho_rime for a clamped density, then
+  // again with qi_rim = qi_tot when the rime exceeds the total; the second
+  // overrides the first, and `lt` excludes both, so one division serves.
+  const bool over = qi_rim > qi_tot && rho_rime > R(0);
+  if (over || clamped) bi_rim = (over ? qi_tot : qi_rim) / rho_rime;
+  if (over) qi_rim = qi_tot;
   if (qi_rim < R(QSMALL)) { qi_rim = R(0); bi_rim = R(0); }
   return rho_rime;
 }

@@ -49,7 +49,6 @@ hipError_t noalloc_scratch_on(int dev, double** out) {
 }  // namespace
 
 
-
 namespace p3x_detail {
 
 int dev_fail(hipError_t e) { return e == hipErrorOutOfMemory ? P3X_ERR_NO_MEMORY : P3X_ERR_DEVICE; }
@@ -62,11 +61,9 @@ int status_of_err(unsigned err) {
 }
 
 // Default launch shape per (precision, nlev), from sweeps on MI355X at 21600x72
-// and 16384x128 in both precisions.  Part 1: 32 columns x 256 threads.  Part 2:
-// compiled for two wavefronts per SIMD (in FP64 that spills a little, which
-// costs less than the latency a single wavefront leaves exposed).
-// Sedimentation: 32 lanes per column up to 160 levels (at most 5 levels per
-// lane; more push the kernel to one wavefront per SIMD).
+// and 16384x128 in both precisions.  Part 1: one thread per (level, column).
+// Part 2: compiled for two wavefronts per SIMD (in FP64 that spills a little,
+// which costs less than the latency a single wavefront leaves exposed).
 void default_shape(int precision, int nk, p3d::Shape* sh) {
   sh->cb = 32;
   sh->nt = 256;
@@ -76,12 +73,17 @@ void default_shape(int precision, int nk, p3d::Shape* sh) {
   // unless an explicit cb x nt shape asks for the block kernel (the batch tuner
   // tries those too).  Part 3 runs in a kernel of its own in every form.
   sh->p1lvl = 1;
-  // Sedimentation lanes per column.  FP64 holds more registers per level, so it
-  // moves to 32 lanes (at most 3 levels per lane at 72 levels) sooner; the mixed
-  // form still runs two wavefronts per SIMD with 5 levels per lane (measured at
-  // 21600 x 72: FP64 1115 us at 32 lanes, 1573 at 16; mixed 638 at 16, 689 at 32).
-  const int nk16 = precision == P3X_MIXED ? 80 : 48;
-  sh->lpc = nk <= nk16 ? 16 : (nk <= 32 * 5 ? 32 : 64);
+  // Sedimentation lanes per column, per species group (kernel trace, us):
+  //                     liquid 16 / 32 / 64     ice 16 / 32 / 64
+  //   FP64  21600 x 72     115 / 184 / 247     766 / 767 / 891
+  //   FP64  16384 x 128    111 /  77 / 105     284 / 184 / 171
+  //   mixed 21600 x 72      63 /  84 / 121     412 / 363 / 427
+  //   mixed 16384 x 128     51 /  49 /  60     131 / 122 / 133
+  // (at most 5 levels per lane keep two wavefronts per SIMD; the batched form
+  // runs both groups on lpc, which its tuner also searches).
+  sh->lpc = nk <= 80 ? 16 : (nk <= 32 * 5 ? 32 : 64);
+  if (precision == P3X_MIXED) sh->lpc_ice = nk <= 32 * 5 ? 32 : 64;
+  else sh->lpc_ice = nk <= 96 ? 32 : 64;
 }
 
 // Offsets (in elements) of the tables in one device allocation; each is a
@@ -153,7 +155,7 @@ int set_geometry(int nj, int nk, int layout, int ld, int ld_iface, p3d::KParams*
     const int l = ld > 0 ? ld : nk;
     const int lf = ld_iface > 0 ? ld_iface : std::max(l, nk + 1);
     if (l < nk || lf < nk + 1) return P3X_ERR_ARG;
-    // the column-major pointwise tiles index points with 32-bit arithmetic
+    // column-major point counts stay below 2^31 (the limit the pointwise launches were built for)
     if ((long long)nj * nk >= (1ll << 31)) return P3X_ERR_ARG;
     k->cs = l; k->ks = 1;
     k->fcs = lf; k->fks = 1;
@@ -482,10 +484,13 @@ int resolve_shape(int precision, int nk_max, int cb, int nt, int lpc, int wpe, p
     sh->wpe = 1;
     sh->p1lvl = 0;
   }
-  if (lpc != 0) sh->lpc = lpc;
+  if (lpc != 0) sh->lpc = sh->lpc_ice = lpc;
   if (wpe != 0) sh->wpe = wpe;
-  if (sh->lpc != 16 && sh->lpc != 32 && sh->lpc != 64) return P3X_ERR_ARG;
-  if (nk_max > sh->lpc * p3d::kMaxLevelsPerLane) return P3X_ERR_ARG;
+  if (sh->wpe != 1 && sh->wpe != 2) return P3X_ERR_ARG;
+  for (int l : {sh->lpc, sh->lpc_ice}) {
+    if (l != 16 && l != 32 && l != 64) return P3X_ERR_ARG;
+    if (nk_max > l * p3d::kMaxLevelsPerLane) return P3X_ERR_ARG;
+  }
   return P3X_OK;
 }
 

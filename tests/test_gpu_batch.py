@@ -93,3 +93,36 @@ def test_tuned_batch_unchanged_results(ctx):
     inps = _inputs()
     got = _run_batch(ctx, inps, p3x.batch_defaults(precision=p3x.P3X_MIXED), tune=True)
     _assert_identical(got, _singles(ctx, inps, p3x.P3X_MIXED))
+
+
+def test_bench_config_mixed_64_sets(ctx):
+    """The bench's batched configuration itself: 64 sets, ncol log-uniform in 256-8192,
+    nlev from {72, 128}, dt from {100, 300, 1800} s (the same draws, seed 4, inputs seeded
+    2000 + set), mixed precision, graph-captured after the autotuner; every set bit for
+    bit its own p3x_main call."""
+    import torch
+    rng = np.random.default_rng(4)
+    nsets = 64
+    ncols = np.round(np.exp(rng.uniform(np.log(256), np.log(8192), nsets))).astype(int)
+    nlevs = rng.choice([72, 128], nsets)
+    dts = rng.choice([100.0, 300.0, 1800.0], nsets)
+    specs = [(int(n), int(k), float(dt)) for n, k, dt in zip(ncols, nlevs, dts)]
+    print(f"bench batch: {nsets} sets, {sum(n * k for n, k, _ in specs)} column-levels", flush=True)
+    inps = [make_columns(n, k, seed=2000 + s) for s, (n, k, _) in enumerate(specs)]
+    devs = [device_fields(inp) for inp in inps]
+    sets = [p3x.column_set(d, n, k, dt, debug=dbg) for (d, dbg), (n, k, dt) in zip(devs, specs)]
+    b = p3x.Batch(ctx, sets, p3x.batch_defaults(precision=p3x.P3X_MIXED, use_graph=1))
+    chosen, us = b.tune(reps=1)
+    print(f"tuned: {chosen.as_dict()} {us:.1f} us", flush=True)
+    b.run(torch.cuda.current_stream().cuda_stream)
+    assert ctx.sync() == 0
+    b.close()
+    got = [to_host(d, dbg) for d, dbg in devs]
+    del devs, sets
+    for s, (inp, (n, k, dt)) in enumerate(zip(inps, specs)):
+        g, gd = got[s]
+        w, wd = run_gpu(ctx, inp, dt, True, False, precision=p3x.P3X_MIXED)[:2]
+        for key in FLOAT_FIELDS:
+            np.testing.assert_array_equal(g[key], w[key], err_msg=f"set {s} ({n}x{k}) {key}")
+        for key in wd:
+            np.testing.assert_array_equal(gd[key], wd[key], err_msg=f"set {s} ({n}x{k}) {key}")

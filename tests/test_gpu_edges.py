@@ -196,9 +196,11 @@ def test_check_values_flags_and_trap(ctx, oracle_tables):
     bounds applied to the oracle's outputs, and without the option nothing is checked."""
     import torch
     from p3_compare import device_fields
+    # column 12 runs the full step (columns 0, 7, 14, ... are dry, 1, 10, 19, ... skip
+    # after part 2, and check_values looks at fully stepped columns only)
     inp = make_columns(128, 72, seed=93, dry_every=7, evap_every=9)
     inp["qv"] = inp["qv"].copy()
-    inp["qv"][60, 10] = 0.05
+    inp["qv"][60, 12] = 0.05
     want, wdbg, _ = oracle.run_oracle(inp, oracle_tables, 300.0, True, False, math=MATH)
     d, dbg = device_fields(inp)
     dbg["check_values"] = torch.full((128,), -7, dtype=torch.int32, device="cuda:0")
@@ -206,7 +208,8 @@ def test_check_values_flags_and_trap(ctx, oracle_tables):
     ctx.main(d, 128, 72, 300.0, True, False, debug=dbg, stream=torch.cuda.current_stream().cuda_stream)
     torch.cuda.synchronize()
     exp = _check_values_expected(inp, want, wdbg)
-    assert exp[10] & 2 and not (exp & 1).any()
+    assert wdbg["col_status"][12] == 2
+    assert exp[12] & 2 and not (exp & 1).any()
     np.testing.assert_array_equal(dbg["check_values"].cpu().numpy(), exp)
 
     hot = make_columns(128, 72, seed=94, dry_every=7, evap_every=9)

@@ -150,3 +150,18 @@ def test_mixed_within_stated_error(ctx, oracle_tables, ncol, nlev, dt):
     np.testing.assert_array_equal(gdbg["col_status"], wdbg["col_status"])
     before, after = water_budget(inp, got, dt)
     assert np.all(np.abs(after - before) <= BUDGET_RTOL * np.abs(before))
+
+
+def test_mixed_full_size_mix(ctx, oracle_tables):
+    """The mixed form on its own workload, 16384 x 128, dt 100 s, against the FP64 oracle:
+    every field within the stated bound, the column status exact, water closed."""
+    inp = make_columns(16384, 128, seed=2)
+    want, wdbg, _ = oracle.run_oracle(inp, oracle_tables, 100.0, True, False, math=MATH)
+    got, gdbg, _ = run_gpu(ctx, inp, 100.0, True, False, precision=p3x.P3X_MIXED)
+    errs = _normalized_errors(got, want)
+    for k, v in sorted(errs.items(), key=lambda kv: -kv[1])[:6]:
+        print(f"mixed 16384x128 dt=100: {k} {v:.3e}")
+    assert max(errs.values()) <= MIXED_BOUND, errs
+    np.testing.assert_array_equal(gdbg["col_status"], wdbg["col_status"])
+    before, after = water_budget(inp, got, 100.0)
+    assert np.all(np.abs(after - before) <= BUDGET_RTOL * np.abs(before))
