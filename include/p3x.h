@@ -379,6 +379,16 @@ int p3x_cld_fraction(p3x_context* ctx, const double* qi, const double* cldfrac_l
                      double* cldfrac_tot, double* cldfrac_ice_4out, double* cldfrac_tot_4out,
                      void* stream);
 
+/* ---- SHOC's post-process, the producer of P3's inv_qc_relvar
+ * (physics/shoc/eamxx_shoc_process_interface.hpp:272-300): inv_qc_relvar =
+ * min(10, max(0.001, qc^2 / qc2)) where qc != 0 and qc2 != 0, else 1 (qc: the
+ * cloud liquid mixing ratio, qc2: SHOC's variance of it; the wet/dry factor
+ * cancels in the ratio), and cldfrac_liq clipped to at most 1 in place (NULL:
+ * left alone).  Pointwise over ncol*nlev dense device arrays in one layout;
+ * enqueued on stream (NULL = default), returns at once. ---- */
+int p3x_shoc_postprocess(p3x_context* ctx, const double* qc, const double* qc2, double* cldfrac_liq,
+                         double* inv_qc_relvar, int ncol, int nlev, void* stream);
+
 /* ---- served form (no reference counterpart): step requests from concurrent
  * ensemble members, continuously batched.  Members register at any time, also
  * while others are being served; their arrays (caller-owned device memory) stay
@@ -427,6 +437,34 @@ int p3x_server_create(p3x_context* ctx, const p3x_runtime* runtime, const p3x_se
  * beside the members already registered, or its scratch can never fit one
  * slot's share of the scratch budget. */
 int p3x_server_add_member(p3x_server* server, const p3x_column_set* set, int* member);
+/* The neighbour step of a served process member (the mac_mic order of the
+ * reference's multi-process runs): before P3, SHOC's post-process (when qc2 is
+ * set: inv_qc_relvar of the process's fields from their qc and qc2, cldfrac_liq
+ * clipped) and CldFraction (when cldfrac_liq is set: the fields' cldfrac_tot from
+ * their qi and cldfrac_liq, plus the three other CldFraction outputs); around
+ * it, the mass-and-energy column conservation check (when mass_err is set: the
+ * column budgets before and after, in budget[4*ncol], and the relative errors
+ * against the step's boundary fluxes, which the fields must then carry).  Every
+ * array is device memory the caller owns, in the process's layout. */
+typedef struct p3x_step_chain {
+  const double* qc2;                        /* SHOC's qc variance; NULL = no SHOC stage        */
+  double* cldfrac_liq;                      /* NULL = no CldFraction stage                     */
+  double ice_threshold, ice_4out_threshold; /* CldFraction thresholds (1e-12, 1e-5 defaults)   */
+  double *cldfrac_ice, *cldfrac_ice_4out, *cldfrac_tot_4out;
+  double *mass_err, *energy_err;            /* ncol each; NULL = no conservation check        */
+  double* budget;                           /* 4*ncol: mass and energy before, then after     */
+  const double *u, *v, *ps, *phis;          /* optional inputs of the energy (NULL = 0)        */
+} p3x_step_chain;
+/* Register a member whose requests each run a whole P3Microphysics step (the
+ * process's pre-amble, p3_main and post-amble, as p3x_process_run) on the
+ * resident field-manager fields, with the optional neighbour stages of chain
+ * (NULL = none).  The process object (p3x_process_create*, on the server's
+ * context) holds the member's process-local arrays and must outlive the server;
+ * one process per member.  Each request's results equal those of running the
+ * same stages alone (the CldFraction, SHOC and budget entry points and
+ * p3x_process_run) on one stream. */
+int p3x_server_add_process(p3x_server* server, p3x_process* process, const p3x_atm_fields* fields,
+                           const p3x_step_chain* chain, int* member);
 /* Queue one step of member with time step dt; thread-safe. */
 int p3x_server_submit(p3x_server* server, int member, double dt, unsigned long long tag);
 /* Move up to max completed requests into out; waits up to wait_ms for one if

@@ -25,7 +25,11 @@ ORACLE_PMATH_SO = os.path.join(HERE, "libp3oracle_pmath.so")
 #              the bit-for-bit comparison with micro_p3.F90;
 #   "portable" scream_amd/csrc/p3_math.h, the functions the HIP FP64 form
 #              uses, for the bit-for-bit comparison with the GPU.
-FLAVOURS = {"glibc": ORACLE_SO, "fortran": ORACLE_FMATH_SO, "portable": ORACLE_PMATH_SO}
+ORACLE_EXPUP_SO = os.path.join(HERE, "libp3oracle_expup.so")
+#   "exp_ulp_up" the glibc flavour with every exp result one ulp up: a
+#              conditioning probe (how far one ulp in one function moves p3_main)
+FLAVOURS = {"glibc": ORACLE_SO, "fortran": ORACLE_FMATH_SO, "portable": ORACLE_PMATH_SO,
+            "exp_ulp_up": ORACLE_EXPUP_SO}
 REF_SO = os.path.join(HERE, "_ref", "libp3ref.so")
 TABLEGEN_SO = {"glibc": os.path.join(HERE, "libp3tablegen.so"),
                "portable": os.path.join(HERE, "libp3tablegen_pmath.so")}
@@ -130,9 +134,11 @@ def load_oracle(math="glibc"):
             raise RuntimeError(f"{so} is not built (run `make -C oracle`)")
         lib = C.CDLL(so)
         lib.p3o_main.restype = C.c_int
+        lib.p3o_main_ex.restype = C.c_int
         lib.p3o_main.argtypes = [C.POINTER(_Runtime), C.POINTER(_Tables), C.POINTER(_Fields),
                                  C.POINTER(_Debug), C.c_int, C.c_int, C.c_double, C.c_int,
                                  C.c_int, C.c_int]
+        lib.p3o_main_ex.argtypes = lib.p3o_main.argtypes + [C.c_int]
         lib.p3o_elementary.restype = C.c_double
         lib.p3o_elementary.argtypes = [C.c_int, C.c_double, C.c_double]
         lib.p3o_qv_sat_dry.restype = C.c_double
@@ -146,6 +152,7 @@ def load_oracle(math="glibc"):
         lib.p3o_column_budget.argtypes = [_dp] * 10 + [C.c_int, C.c_int, _dp, _dp]
         lib.p3o_conservation_errors.argtypes = [_dp] * 8 + [C.c_int, C.c_double, _dp, _dp]
         lib.p3o_cld_fraction.argtypes = [_dp, _dp, C.c_int, C.c_int, C.c_double, C.c_double] + [_dp] * 4
+        lib.p3o_shoc_postprocess.argtypes = [_dp, _dp, _dp, _dp, C.c_int, C.c_int]
         lib.p3o_preamble.argtypes = [C.POINTER(_Atm), C.c_int, C.c_int]
         lib.p3o_postamble.argtypes = [C.POINTER(_Atm), C.c_int, C.c_int, C.c_double]
         lib.p3o_upwind_step.argtypes = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_int,
@@ -179,8 +186,9 @@ def _tables(tab):
 
 
 def run_oracle(inputs: dict, tables: dict, dt: float, predict_nc=True, prescribed_ccn=False,
-               runtime=None, nthreads=0, math="glibc"):
-    """p3_main on copies of ``inputs``; returns (fields dict, debug dict, n_errors)."""
+               runtime=None, nthreads=0, math="glibc", small_kernel_exits=False):
+    """p3_main on copies of ``inputs``; returns (fields dict, debug dict, n_errors).
+    small_kernel_exits: the early exits of the reference's small-kernel dispatch."""
     lib = load_oracle(math)
     nlev, ncol = inputs["qc"].shape
     f = {k: np.array(inputs[k], dtype=np.float64, order="C") for k in STATE + INPUTS}
@@ -198,8 +206,9 @@ def run_oracle(inputs: dict, tables: dict, dt: float, predict_nc=True, prescribe
                                                          "ice_table_idx")])
     rt = _runtime(runtime)
     tb, keep = _tables(tables)
-    nerr = lib.p3o_main(C.byref(rt), C.byref(tb), C.byref(fs), C.byref(ds), ncol, nlev,
-                        float(dt), int(predict_nc), int(prescribed_ccn), int(nthreads))
+    nerr = lib.p3o_main_ex(C.byref(rt), C.byref(tb), C.byref(fs), C.byref(ds), ncol, nlev,
+                           float(dt), int(predict_nc), int(prescribed_ccn), int(nthreads),
+                           int(bool(small_kernel_exits)))
     del keep
     return f, dbg, nerr
 
@@ -296,6 +305,20 @@ def cld_fraction(qi, cldfrac_liq, ice_threshold=1e-12, ice_4out_threshold=1e-5, 
     lib.p3o_cld_fraction(_ptr(qi), _ptr(liq), ncol, nlev, float(ice_threshold), float(ice_4out_threshold),
                          *[_ptr(out[k]) for k in CLD_FRACTION_OUT])
     return out
+
+
+def shoc_postprocess(qc, qc2, cldfrac_liq=None, math="glibc"):
+    """SHOC's post-process (p3o_shoc_postprocess) on (nlev, ncol) arrays: returns
+    (inv_qc_relvar, the clipped cldfrac_liq or None)."""
+    lib = load_oracle(math)
+    qc = np.ascontiguousarray(qc, dtype=np.float64)
+    qc2 = np.ascontiguousarray(qc2, dtype=np.float64)
+    assert qc.shape == qc2.shape and qc.ndim == 2
+    nlev, ncol = qc.shape
+    liq = None if cldfrac_liq is None else np.array(cldfrac_liq, dtype=np.float64, order="C")
+    r = np.zeros_like(qc)
+    lib.p3o_shoc_postprocess(_ptr(qc), _ptr(qc2), None if liq is None else _ptr(liq), _ptr(r), ncol, nlev)
+    return r, liq
 
 
 def conservation_errors(m0, e0, m1, e1, vapor, water, ice, heat, dt, math="glibc"):

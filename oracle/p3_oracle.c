@@ -101,6 +101,14 @@ static const double lookup_table_1a_dum1_c = 4.135985029041767e+00;
 #elif defined(P3O_FORTRAN_MATH)
 #define P3_CBRT(x) pow((x), 1.0 / 3.0)
 #define P3_EXPM1(x) (exp(x) - 1)
+#elif defined(P3O_EXP_ULP_UP)
+/* Conditioning probe (tests only): the glibc flavour with every exp result
+ * moved one ulp up, to measure how far a one-ulp change in one elementary
+ * function moves p3_main's outputs. */
+static double p3o_exp_ulp_up(double x) { return nextafter(exp(x), INFINITY); }
+#define exp(x) p3o_exp_ulp_up(x)
+#define P3_CBRT(x) cbrt(x)
+#define P3_EXPM1(x) expm1(x)
 #else
 #define P3_CBRT(x) cbrt(x)
 #define P3_EXPM1(x) expm1(x)
@@ -527,8 +535,16 @@ typedef struct col_io {
 
 /* ========================= p3_main on one column ========================= */
 
+/* small_kernel_exits: the early exits of the reference's small-kernel dispatch
+ * (disp/p3_main_impl_part2_disp.cpp:106-130, disp/p3_cloud_sed_impl_disp.cpp:46,
+ * disp/p3_rain_sed_impl_disp.cpp:42, disp/p3_ice_sed_impl_disp.cpp:42,85,
+ * disp/p3_main_impl_part3_disp.cpp:68): a column leaves only when
+ * !(nucleationPossible || hydrometeorsPresent); a column whose hydrometeors part 2
+ * removed still runs sedimentation, homogeneous freezing and part 3.  Otherwise
+ * the monolithic exits (impl/p3_main_impl.hpp:223-228,258). */
 static void p3_column(const p3o_runtime* rt, const p3o_tables* tb, int nk, double dt,
-                      int predictNc, int prescribedCCN, col_io* c, col_ws* w, int* err) {
+                      int predictNc, int prescribedCCN, int small_kernel_exits, col_io* c, col_ws* w,
+                      int* err) {
   const double inv_dt = 1 / dt;
   const int kbot = nk - 1; /* ktop = 0 */
   const int do_ice_production = rt->do_ice_production;
@@ -1178,8 +1194,8 @@ static void p3_column(const p3o_runtime* rt, const p3o_tables* tb, int nk, doubl
   }
 
   c->status = 1;
-  if (!hydrometeorsPresent) return;
-  c->status = 2;
+  if (!hydrometeorsPresent && !small_kernel_exits) return;
+  if (hydrometeorsPresent) c->status = 2;
 
   /* ---- cloud_sedimentation, P3I/p3_cloud_sed_impl.hpp:14-147 ---- */
   {
@@ -1460,6 +1476,12 @@ void p3o_runtime_defaults(p3o_runtime* rt) {
 int p3o_main(const p3o_runtime* rt, const p3o_tables* tab, const p3o_fields* f,
              const p3o_debug* dbg, int ncol, int nlev, double dt,
              int predictNc, int prescribedCCN, int nthreads) {
+  return p3o_main_ex(rt, tab, f, dbg, ncol, nlev, dt, predictNc, prescribedCCN, nthreads, 0);
+}
+
+int p3o_main_ex(const p3o_runtime* rt, const p3o_tables* tab, const p3o_fields* f,
+                const p3o_debug* dbg, int ncol, int nlev, double dt,
+                int predictNc, int prescribedCCN, int nthreads, int small_kernel_exits) {
   int nerr = 0;
   const int nk = nlev;
 #ifdef _OPENMP
@@ -1505,7 +1527,7 @@ int p3o_main(const p3o_runtime* rt, const p3o_tables* tab, const p3o_fields* f,
       }
       c.ice_idx = ice_idx;
       int err = 0;
-      p3_column(rt, tab, nk, dt, predictNc, prescribedCCN, &c, &w, &err);
+      p3_column(rt, tab, nk, dt, predictNc, prescribedCCN, small_kernel_exits, &c, &w, &err);
       nerr += err;
       /* scatter */
       double* const out_dst[] = {f->qc, f->nc, f->qr, f->nr, f->qi, f->qm, f->ni, f->bm, f->qv, f->th,
@@ -1640,6 +1662,25 @@ void p3o_cld_fraction(const double* qi, const double* cldfrac_liq, int ncol, int
     cldfrac_ice_4out[x] = ice4;
     cldfrac_tot[x] = ice < cldfrac_liq[x] ? cldfrac_liq[x] : ice;
     cldfrac_tot_4out[x] = ice4 < cldfrac_liq[x] ? cldfrac_liq[x] : ice4;
+  }
+}
+
+/* SHOCPostprocess's P3 inputs (components/eamxx/src/physics/shoc/
+ * eamxx_shoc_process_interface.hpp:288-300): inv_qc_relvar = min(10,
+ * max(0.001, qc^2/qc2)) where qc != 0 and qc2 != 0, else 1; cldfrac_liq =
+ * min(cldfrac_liq, 1) when given.  ekat's min/max as std::min/std::max. */
+void p3o_shoc_postprocess(const double* qc, const double* qc2, double* cldfrac_liq, double* inv_qc_relvar,
+                          int ncol, int nlev) {
+  const size_t n = (size_t)ncol * nlev;
+  for (size_t x = 0; x < n; ++x) {
+    double r = 1;
+    if (qc[x] != 0 && qc2[x] != 0) {
+      const double q = qc[x] * qc[x] / qc2[x];
+      const double lo = 0.001 < q ? q : 0.001;
+      r = lo < 10.0 ? lo : 10.0;
+    }
+    inv_qc_relvar[x] = r;
+    if (cldfrac_liq) cldfrac_liq[x] = 1.0 < cldfrac_liq[x] ? 1.0 : cldfrac_liq[x];
   }
 }
 

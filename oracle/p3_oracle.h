@@ -130,6 +130,11 @@ void p3o_cld_fraction(const double* qi, const double* cldfrac_liq, int ncol, int
                       double ice_4out_threshold, double* cldfrac_ice, double* cldfrac_tot,
                       double* cldfrac_ice_4out, double* cldfrac_tot_4out);
 
+/* SHOC's post-process that produces P3's inv_qc_relvar (and clips cldfrac_liq,
+ * which may be NULL).  Dense arrays of ncol*nlev points. */
+void p3o_shoc_postprocess(const double* qc, const double* qc2, double* cldfrac_liq, double* inv_qc_relvar,
+                          int ncol, int nlev);
+
 /* Defaults of P3Runtime. */
 void p3o_runtime_defaults(p3o_runtime* rt);
 
@@ -139,6 +144,12 @@ void p3o_runtime_defaults(p3o_runtime* rt);
 int p3o_main(const p3o_runtime* rt, const p3o_tables* tab, const p3o_fields* f,
              const p3o_debug* dbg, int ncol, int nlev, double dt,
              int predictNc, int prescribedCCN, int nthreads);
+/* p3o_main with the early exits of the reference's small-kernel dispatch
+ * (small_kernel_exits = 1: a column whose hydrometeors part 2 removed still runs
+ * sedimentation, homogeneous freezing and part 3) or the monolithic ones (0). */
+int p3o_main_ex(const p3o_runtime* rt, const p3o_tables* tab, const p3o_fields* f,
+                const p3o_debug* dbg, int ncol, int nlev, double dt,
+                int predictNc, int prescribedCCN, int nthreads, int small_kernel_exits);
 
 /* Building blocks exported for unit tests. */
 double p3o_qv_sat_dry(double t, double p, int ice);

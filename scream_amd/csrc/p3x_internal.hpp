@@ -65,6 +65,29 @@ bool fields_present(const p3x_prognostic_state* ps, const p3x_diagnostic_inputs*
 // Bytes of scratch one column set of nj columns x nk levels needs, rounded up to 256.
 size_t scratch_bytes_for(int nj, int nk);
 
+// The p3_main arguments of one P3Microphysics process step (p3x_process.hip).
+struct ProcessArgs {
+  p3x_launch o;
+  p3x_prognostic_state ps;
+  p3x_diagnostic_inputs di;
+  p3x_diagnostic_outputs dout;
+  p3x_history_only hist;
+  p3x_infrastructure inf;
+  p3d::Amble amble;
+};
+int process_args(p3x_process* p, const p3x_atm_fields* f, double dt, const p3x_launch* opts, ProcessArgs* out);
+// KParams of one process step (no scratch, no debug arrays): p3_main on the
+// process's views with the pre/post-amble fused.
+int process_kparams(p3x_process* p, const p3x_atm_fields* f, double dt, p3d::KParams* k);
+// Bytes of the process's own device arrays, and its context.
+size_t process_local_bytes(const p3x_process* p);
+p3x_context* process_context(const p3x_process* p);
+// The neighbour stages of a served process step (p3x_step_chain): argument
+// check, and the stages before and after the step's kernels on stream s.
+int chain_check(const p3x_process* p, const p3x_atm_fields* f, const p3x_step_chain* c);
+int chain_before(const p3x_process* p, const p3x_atm_fields* f, const p3x_step_chain* c, hipStream_t s);
+int chain_after(const p3x_process* p, const p3x_atm_fields* f, const p3x_step_chain* c, double dt, hipStream_t s);
+
 // Launch shape for a batch whose deepest set has nk_max levels: the defaults,
 // overridden by the nonzero options.  P3X_ERR_ARG if not compiled in.
 int resolve_shape(int precision, int nk_max, int cb, int nt, int lpc, int wpe, p3d::Shape* sh);

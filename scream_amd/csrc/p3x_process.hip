@@ -44,6 +44,34 @@ __global__ __launch_bounds__(256) void p3_cld_fraction_kernel(const CldFracParam
   a.tot4[x] = ice4 < liq ? liq : ice4;
 }
 
+struct ShocPostParams {
+  const double *qc, *qc2;
+  double *cldfrac_liq, *inv_qc_relvar;
+  size_t n;
+};
+
+// SHOCPostprocess's P3 inputs, eamxx_shoc_process_interface.hpp:288-300: the
+// relative-variance ratio clamped to [0.001, 10] (1 where qc or its variance is
+// 0) and the liquid cloud fraction clipped to at most 1; one thread per point.
+// ekat's scalar min/max are taken as std::min/std::max (they differ only for a
+// NaN argument, which no test feeds).
+__global__ __launch_bounds__(256) void p3_shoc_post_kernel(const ShocPostParams a) {
+  const size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= a.n) return;
+  const double qc = a.qc[x], qc2 = a.qc2[x];
+  double r = 1;
+  if (qc != 0 && qc2 != 0) {
+    const double q = qc * qc / qc2;
+    const double lo = 0.001 < q ? q : 0.001;  // max(0.001, q)
+    r = lo < 10.0 ? lo : 10.0;                 // min(10, .)
+  }
+  a.inv_qc_relvar[x] = r;
+  if (a.cldfrac_liq) {
+    const double c = a.cldfrac_liq[x];
+    a.cldfrac_liq[x] = 1.0 < c ? 1.0 : c;       // min(c, 1)
+  }
+}
+
 struct BudgetParams {
   p3x_budget_fields f;
   double *mass, *energy;
@@ -104,6 +132,15 @@ extern "C" int p3x_column_budget(p3x_context* c, const p3x_budget_fields* f, int
   return hipGetLastError() == hipSuccess ? P3X_OK : P3X_ERR_DEVICE;
 }
 
+extern "C" int p3x_shoc_postprocess(p3x_context* c, const double* qc, const double* qc2, double* cldfrac_liq,
+                                    double* inv_qc_relvar, int ncol, int nlev, void* stream) {
+  if (!c || !qc || !qc2 || !inv_qc_relvar || ncol <= 0 || nlev <= 0) return P3X_ERR_ARG;
+  const ShocPostParams a{qc, qc2, cldfrac_liq, inv_qc_relvar, (size_t)ncol * nlev};
+  const size_t blocks = (a.n + 255) / 256;
+  hipLaunchKernelGGL(p3_shoc_post_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, a);
+  return hipGetLastError() == hipSuccess ? P3X_OK : P3X_ERR_DEVICE;
+}
+
 extern "C" int p3x_cld_fraction(p3x_context* c, const double* qi, const double* cldfrac_liq, int ncol,
                                 int nlev, double ice_threshold, double ice_4out_threshold,
                                 double* cldfrac_ice, double* cldfrac_tot, double* cldfrac_ice_4out,
@@ -139,6 +176,7 @@ struct p3x_process {
   double* mem = nullptr;  // every process-local array below
   double *inv_exner, *th_atm, *cld_frac_l, *cld_frac_i, *dz, *qv2qi_depos_tend, *rho_qi;
   double *precip_liq_flux, *precip_ice_flux, *precip_liq_surf, *precip_ice_surf, *unused;
+  size_t local_bytes = 0;  // of mem
 };
 
 extern "C" int p3x_process_create(p3x_context* c, const p3x_runtime* rt, int ncol, int nlev, int predictNc,
@@ -166,6 +204,7 @@ extern "C" int p3x_process_create_layout(p3x_context* c, const p3x_runtime* rt, 
   const size_t lev = layout == P3X_LEVEL_MAJOR ? (size_t)nlev * g.ks : (size_t)ncol * g.cs;
   const size_t iface = layout == P3X_LEVEL_MAJOR ? (size_t)(nlev + 1) * g.fks : (size_t)ncol * g.fcs;
   const size_t total = 8 * lev + 2 * iface + 2 * (size_t)ncol;
+  p->local_bytes = total * sizeof(double);
   const hipError_t e = hipMalloc((void**)&p->mem, total * sizeof(double));
   if (e != hipSuccess) {
     delete p;
@@ -195,7 +234,12 @@ extern "C" int p3x_process_create_layout(p3x_context* c, const p3x_runtime* rt, 
   return P3X_OK;
 }
 
-extern "C" int p3x_process_run(p3x_process* p, const p3x_atm_fields* f, double dt, const p3x_launch* opts) {
+// The p3_main arguments of one process step on the field-manager fields f (the
+// process's views, eamxx_p3_process_interface.cpp:284-337) and the fused
+// pre/post-amble; the launch options must name the process's own layout (or
+// none).  P3X_ERR_ARG for a missing field or another layout.
+int p3x_detail::process_args(p3x_process* p, const p3x_atm_fields* f, double dt, const p3x_launch* opts,
+                             ProcessArgs* out) {
   if (!p || !f || !(dt > 0)) return P3X_ERR_ARG;
   const void* must[] = {f->p_mid, f->p_dry_mid, f->pseudo_density, f->pseudo_density_dry, f->cldfrac_tot,
                         f->nc_nuceat_tend, f->ni_activated, f->inv_qc_relvar, f->T_mid, f->qv, f->qc, f->nc,
@@ -209,33 +253,31 @@ extern "C" int p3x_process_run(p3x_process* p, const p3x_atm_fields* f, double d
   const bool some_flux = f->vapor_flux || f->water_flux || f->ice_flux || f->heat_flux;
   const bool all_flux = f->vapor_flux && f->water_flux && f->ice_flux && f->heat_flux;
   if (some_flux && !all_flux) return P3X_ERR_ARG;
-  p3x_launch o;
-  if (opts) o = *opts;
-  else p3x_launch_defaults(&o);
+  ProcessArgs& r = *out;
+  if (opts) r.o = *opts;
+  else p3x_launch_defaults(&r.o);
   // the layout is the process's own (its local arrays were allocated in it)
-  if ((o.layout != P3X_LEVEL_MAJOR && o.layout != p->layout) || (o.ld != 0 && o.ld != p->ld) ||
-      (o.ld_iface != 0 && o.ld_iface != p->ld_iface))
+  if ((r.o.layout != P3X_LEVEL_MAJOR && r.o.layout != p->layout) || (r.o.ld != 0 && r.o.ld != p->ld) ||
+      (r.o.ld_iface != 0 && r.o.ld_iface != p->ld_iface))
     return P3X_ERR_ARG;
-  o.layout = p->layout;
-  o.ld = p->ld;
-  o.ld_iface = p->ld_iface;
-  hipStream_t s = (hipStream_t)o.stream;
-
+  r.o.layout = p->layout;
+  r.o.ld = p->ld;
+  r.o.ld_iface = p->ld_iface;
   // p3_main on the process's views (eamxx_p3_process_interface.cpp:284-337); the
   // pre-amble runs inside part 1 and the post-amble inside part 3.
-  p3x_prognostic_state ps = {f->qc, f->nc, f->qr, f->nr, f->qi, f->qm, f->ni, f->bm, f->qv, p->th_atm};
-  p3x_diagnostic_inputs di = {f->nc_nuceat_tend, p->prescribedCCN ? f->nccn : p->unused, f->ni_activated,
-                              f->inv_qc_relvar, p->cld_frac_i, p->cld_frac_l, f->rainfrac, f->p_dry_mid,
-                              p->dz, f->pseudo_density_dry, p->inv_exner, f->qv_prev_micro_step,
-                              f->T_prev_micro_step};
-  p3x_diagnostic_outputs dout = {p->qv2qi_depos_tend, p->precip_liq_surf, p->precip_ice_surf,
-                                 f->eff_radius_qc, f->eff_radius_qi, f->eff_radius_qr, p->rho_qi,
-                                 p->precip_liq_flux, p->precip_ice_flux, f->precip_total_tend, f->nevapr};
-  p3x_history_only hist = {f->micro_liq_ice_exchange, f->micro_vap_liq_exchange, f->micro_vap_ice_exchange};
+  r.ps = {f->qc, f->nc, f->qr, f->nr, f->qi, f->qm, f->ni, f->bm, f->qv, p->th_atm};
+  r.di = {f->nc_nuceat_tend, p->prescribedCCN ? f->nccn : p->unused, f->ni_activated,
+          f->inv_qc_relvar, p->cld_frac_i, p->cld_frac_l, f->rainfrac, f->p_dry_mid,
+          p->dz, f->pseudo_density_dry, p->inv_exner, f->qv_prev_micro_step,
+          f->T_prev_micro_step};
+  r.dout = {p->qv2qi_depos_tend, p->precip_liq_surf, p->precip_ice_surf,
+            f->eff_radius_qc, f->eff_radius_qi, f->eff_radius_qr, p->rho_qi,
+            p->precip_liq_flux, p->precip_ice_flux, f->precip_total_tend, f->nevapr};
+  r.hist = {f->micro_liq_ice_exchange, f->micro_vap_liq_exchange, f->micro_vap_ice_exchange};
   // run_impl zeroes the three exchange fields before p3_main (eamxx_p3_run.cpp:27-29)
-  p3x_infrastructure inf = {dt, ++p->it, 0, p->ncol - 1, 0, p->nlev - 1, p->predictNc, p->prescribedCCN,
-                            nullptr, 1};
-  p3d::Amble a{};
+  r.inf = {dt, p->it + 1, 0, p->ncol - 1, 0, p->nlev - 1, p->predictNc, p->prescribedCCN, nullptr, 1};
+  p3d::Amble& a = r.amble;
+  a = p3d::Amble{};
   a.on = 1;
   a.p_mid = f->p_mid;
   a.dp_wet = f->pseudo_density;
@@ -254,12 +296,81 @@ extern "C" int p3x_process_run(p3x_process* p, const p3x_atm_fields* f, double d
   a.water_flux = f->water_flux;
   a.ice_flux = f->ice_flux;
   a.heat_flux = f->heat_flux;
-  p3x_launch mo = o;
-  mo.async = 1;
-  const int st = main_impl(p->ctx, &p->rt, &ps, &di, &dout, &inf, &hist, p->ncol, p->nlev, &mo, nullptr, &a);
+  return P3X_OK;
+}
+
+int p3x_detail::process_kparams(p3x_process* p, const p3x_atm_fields* f, double dt, p3d::KParams* k) {
+  ProcessArgs r;
+  int st = process_args(p, f, dt, nullptr, &r);
   if (st != P3X_OK) return st;
-  if (o.async) return P3X_OK;
+  st = fill_params(p->ctx, &p->rt, &r.ps, &r.di, &r.dout, &r.inf, &r.hist, p->ncol, p->nlev, r.o.layout, r.o.ld,
+                   r.o.ld_iface, nullptr, nullptr, nullptr, nullptr, k);
+  if (st != P3X_OK) return st;
+  k->amble = r.amble;
+  return P3X_OK;
+}
+
+size_t p3x_detail::process_local_bytes(const p3x_process* p) { return p->local_bytes; }
+p3x_context* p3x_detail::process_context(const p3x_process* p) { return p->ctx; }
+
+extern "C" int p3x_process_run(p3x_process* p, const p3x_atm_fields* f, double dt, const p3x_launch* opts) {
+  ProcessArgs r;
+  const int st0 = process_args(p, f, dt, opts, &r);
+  if (st0 != P3X_OK) return st0;
+  ++p->it;
+  p3x_launch mo = r.o;
+  mo.async = 1;
+  const int st = main_impl(p->ctx, &p->rt, &r.ps, &r.di, &r.dout, &r.inf, &r.hist, p->ncol, p->nlev, &mo, nullptr,
+                           &r.amble);
+  if (st != P3X_OK) return st;
+  if (r.o.async) return P3X_OK;
   return p3x_sync(p->ctx);
+}
+
+// The neighbour stages of a served process step (include/p3x.h, p3x_step_chain),
+// enqueued on s around the step's kernels in the order of MicrophysicsStep:
+// SHOC's post-process, CldFraction and the first column budget before; the
+// second budget and the conservation errors after.
+int p3x_detail::chain_check(const p3x_process* p, const p3x_atm_fields* f, const p3x_step_chain* c) {
+  if (!c) return P3X_OK;
+  const bool dense = p->layout == P3X_LEVEL_MAJOR ? p->ld == p->ncol : p->ld == p->nlev;
+  if ((c->qc2 || c->cldfrac_liq) && !dense) return P3X_ERR_ARG;  // the pointwise stages take dense arrays
+  if (c->qc2 && !f->inv_qc_relvar) return P3X_ERR_ARG;
+  if (c->cldfrac_liq && (!c->cldfrac_ice || !c->cldfrac_ice_4out || !c->cldfrac_tot_4out)) return P3X_ERR_ARG;
+  if (c->mass_err && (!c->energy_err || !c->budget || !f->vapor_flux)) return P3X_ERR_ARG;
+  return P3X_OK;
+}
+
+int p3x_detail::chain_before(const p3x_process* p, const p3x_atm_fields* f, const p3x_step_chain* c, hipStream_t s) {
+  if (!c) return P3X_OK;
+  int st = P3X_OK;
+  if (c->qc2)
+    st = p3x_shoc_postprocess(p->ctx, f->qc, c->qc2, c->cldfrac_liq, const_cast<double*>(f->inv_qc_relvar),
+                              p->ncol, p->nlev, s);
+  if (st == P3X_OK && c->cldfrac_liq)
+    st = p3x_cld_fraction(p->ctx, f->qi, c->cldfrac_liq, p->ncol, p->nlev, c->ice_threshold, c->ice_4out_threshold,
+                          c->cldfrac_ice, const_cast<double*>(f->cldfrac_tot), c->cldfrac_ice_4out,
+                          c->cldfrac_tot_4out, s);
+  if (st == P3X_OK && c->mass_err) {
+    const p3x_budget_fields b{f->pseudo_density, f->T_mid, c->u, c->v, f->qv, f->qc, f->qi, f->qr, c->ps, c->phis,
+                              p->layout, p->ld};
+    st = p3x_column_budget(p->ctx, &b, p->ncol, p->nlev, c->budget, c->budget + p->ncol, s);
+  }
+  return st;
+}
+
+int p3x_detail::chain_after(const p3x_process* p, const p3x_atm_fields* f, const p3x_step_chain* c, double dt,
+                            hipStream_t s) {
+  if (!c || !c->mass_err) return P3X_OK;
+  const int n = p->ncol;
+  const p3x_budget_fields b{f->pseudo_density, f->T_mid, c->u, c->v, f->qv, f->qc, f->qi, f->qr, c->ps, c->phis,
+                            p->layout, p->ld};
+  int st = p3x_column_budget(p->ctx, &b, n, p->nlev, c->budget + 2 * n, c->budget + 3 * n, s);
+  if (st == P3X_OK)
+    st = p3x_conservation_errors(p->ctx, c->budget, c->budget + n, c->budget + 2 * n, c->budget + 3 * n,
+                                 f->vapor_flux, f->water_flux, f->ice_flux, f->heat_flux, n, dt, c->mass_err,
+                                 c->energy_err, s);
+  return st;
 }
 
 extern "C" void p3x_process_destroy(p3x_process* p) {

@@ -73,6 +73,12 @@ struct p3x_server {
     size_t scratch;   // scratch bytes of one step
     size_t resident;  // bytes its device arrays span
     bool busy;
+    // a process member (p3x_server_add_process): its process, fields and
+    // neighbour stages (has_chain: chain holds them)
+    p3x_process* proc = nullptr;
+    p3x_atm_fields fields{};
+    p3x_step_chain chain{};
+    bool has_chain = false;
   };
   std::vector<Member> members;
   double resident_bytes = 0;  // of all members
@@ -166,8 +172,22 @@ int launch_slot(p3x_server* srv, Slot& sl) {
     sc.nsets = lc.nsets;
     sc.grid = grid_for(srv->ctx->ncu, 0, lc.total, srv->occ_sed[lc.L]);
   }
+  // the neighbour stages of process members before their steps ...
+  for (int i = 0; i < n; ++i) {
+    const p3x_server::Member& m = srv->members[sl.reqs[i].member];
+    if (!m.has_chain) continue;
+    st = chain_before(m.proc, &m.fields, &m.chain, sl.stream);
+    if (st != P3X_OK) return st;
+  }
   hipError_t e = hipMemcpyAsync(sl.d_desc, h, bytes, hipMemcpyHostToDevice, sl.stream);
   if (e == hipSuccess) e = launch_plan(srv->opt.precision, plan, sl.stream);
+  // ... and after them
+  for (int i = 0; i < n && e == hipSuccess; ++i) {
+    const p3x_server::Member& m = srv->members[sl.reqs[i].member];
+    if (!m.has_chain) continue;
+    st = chain_after(m.proc, &m.fields, &m.chain, sl.reqs[i].dt, sl.stream);
+    if (st != P3X_OK) return st;
+  }
   if (e == hipSuccess)
     e = hipMemcpyAsync(sl.h_err, d_err, (size_t)n * sizeof(unsigned), hipMemcpyDeviceToHost, sl.stream);
   if (e == hipSuccess) e = hipEventRecord(sl.done, sl.stream);
@@ -379,6 +399,38 @@ extern "C" int p3x_server_add_member(p3x_server* s, const p3x_column_set* set, i
   m.busy = false;
   // registration may happen while the scheduler serves others: the members are
   // only touched under the lock
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (s->resident_bytes + (double)m.resident + s->scratch_total > s->opt.hbm_budget_bytes) return P3X_ERR_BUDGET;
+  s->resident_bytes += (double)m.resident;
+  s->members.push_back(m);
+  *member = (int)s->members.size() - 1;
+  return P3X_OK;
+}
+
+extern "C" int p3x_server_add_process(p3x_server* s, p3x_process* proc, const p3x_atm_fields* fields,
+                                      const p3x_step_chain* chain, int* member) {
+  if (!s || !proc || !fields || !member) return P3X_ERR_ARG;
+  p3x_server::Member m;
+  int st = process_kparams(proc, fields, 1.0, &m.k);  // dt is replaced per request
+  if (st != P3X_OK) return st;
+  if (process_context(proc) != s->ctx) return P3X_ERR_ARG;  // the batch shares one context's tables
+  if (p3d::sed_levels_per_lane(m.k.nlev, s->shape.lpc) == 0) return P3X_ERR_ARG;
+  st = chain_check(proc, fields, chain);
+  if (st != P3X_OK) return st;
+  const size_t need = scratch_bytes_for(m.k.ncol, m.k.nlev);
+  if (need > s->slots[0].arena_bytes) return P3X_ERR_BUDGET;
+  m.proc = proc;
+  m.fields = *fields;
+  if (chain) {
+    m.chain = *chain;
+    m.has_chain = true;
+  }
+  m.scratch = need;
+  // the field-manager fields p3_main sees, the process's own arrays and the chain's
+  const size_t lev = level_span(m.k);
+  m.resident = member_bytes(m.k) + process_local_bytes(proc) +
+               (chain ? (5 * lev + 6 * (size_t)m.k.ncol) * sizeof(double) : 0);
+  m.busy = false;
   std::lock_guard<std::mutex> lk(s->mu);
   if (s->resident_bytes + (double)m.resident + s->scratch_total > s->opt.hbm_budget_bytes) return P3X_ERR_BUDGET;
   s->resident_bytes += (double)m.resident;

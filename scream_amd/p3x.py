@@ -47,11 +47,12 @@ EXPORTS = ("p3x_runtime_defaults", "p3x_launch_defaults", "p3x_tables_load", "p3
            "p3x_finalize", "p3x_main", "p3x_main_noalloc", "p3x_sync", "p3x_last_kernel_times", "p3x_strerror",
            "p3x_batch_defaults",
            "p3x_batch_create", "p3x_batch_run", "p3x_batch_tune", "p3x_batch_destroy",
-           "p3x_server_defaults", "p3x_server_create", "p3x_server_add_member", "p3x_server_submit",
+           "p3x_server_defaults", "p3x_server_create", "p3x_server_add_member", "p3x_server_add_process",
+           "p3x_server_submit",
            "p3x_server_poll", "p3x_server_drain", "p3x_server_stats_get", "p3x_server_destroy",
            "p3x_process_create", "p3x_process_create_layout", "p3x_process_run", "p3x_process_destroy",
            "p3x_column_budget",
-           "p3x_conservation_errors", "p3x_cld_fraction", "p3x_ice_table_generate",
+           "p3x_conservation_errors", "p3x_cld_fraction", "p3x_shoc_postprocess", "p3x_ice_table_generate",
            "p3x_ice_table_write")
 
 # p3x_atm_fields, in struct order (the P3Microphysics field names)
@@ -157,6 +158,15 @@ class AtmFields(C.Structure):
     _fields_ = [(k, C.c_void_p) for k in ATM_FIELDS]
 
 
+class StepChain(C.Structure):
+    """p3x_step_chain: the neighbour stages of a served process member."""
+    _fields_ = [("qc2", C.c_void_p), ("cldfrac_liq", C.c_void_p), ("ice_threshold", C.c_double),
+                ("ice_4out_threshold", C.c_double), ("cldfrac_ice", C.c_void_p),
+                ("cldfrac_ice_4out", C.c_void_p), ("cldfrac_tot_4out", C.c_void_p),
+                ("mass_err", C.c_void_p), ("energy_err", C.c_void_p), ("budget", C.c_void_p),
+                ("u", C.c_void_p), ("v", C.c_void_p), ("ps", C.c_void_p), ("phis", C.c_void_p)]
+
+
 class P3Error(RuntimeError):
     def __init__(self, status: int, what: str = ""):
         self.status = status
@@ -224,6 +234,11 @@ def load() -> C.CDLL:
         lib.p3x_server_create.restype = C.c_int
         lib.p3x_server_add_member.argtypes = [C.c_void_p, C.POINTER(ColumnSet), C.POINTER(C.c_int)]
         lib.p3x_server_add_member.restype = C.c_int
+        lib.p3x_server_add_process.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(AtmFields),
+                                               C.POINTER(StepChain), C.POINTER(C.c_int)]
+        lib.p3x_server_add_process.restype = C.c_int
+        lib.p3x_shoc_postprocess.argtypes = [C.c_void_p] + [C.c_void_p] * 4 + [C.c_int, C.c_int, C.c_void_p]
+        lib.p3x_shoc_postprocess.restype = C.c_int
         lib.p3x_server_submit.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_ulonglong]
         lib.p3x_server_submit.restype = C.c_int
         lib.p3x_server_poll.argtypes = [C.c_void_p, C.POINTER(Completion), C.c_int, C.c_int]
@@ -498,6 +513,24 @@ class Server:
             raise P3Error(st, "p3x_server_add_member")
         return m.value
 
+    def add_process(self, proc: "P3Microphysics", fields: dict, chain: dict | None = None) -> int:
+        """Register a process member: each request runs proc's whole step (pre-amble,
+        p3_main, post-amble) on ``fields`` (ATM_FIELDS names), with the neighbour stages
+        of ``chain`` (StepChain field names mapped to device arrays or thresholds)."""
+        af = AtmFields(*[_addr(fields.get(k)) for k in ATM_FIELDS])
+        ch = None
+        if chain is not None:
+            ch = StepChain(ice_threshold=1e-12, ice_4out_threshold=1e-5)
+            for k, v in chain.items():
+                setattr(ch, k, v if isinstance(v, float) else _addr(v))
+        self._sets.append((af, ch, proc, fields, chain))
+        m = C.c_int(-1)
+        st = self._lib.p3x_server_add_process(self.handle, proc.handle, C.byref(af),
+                                              C.byref(ch) if ch is not None else None, C.byref(m))
+        if st != P3X_OK:
+            raise P3Error(st, "p3x_server_add_process")
+        return m.value
+
     def submit(self, member: int, dt: float, tag: int = 0):
         st = self._lib.p3x_server_submit(self.handle, int(member), float(dt), int(tag))
         if st != P3X_OK:
@@ -648,6 +681,16 @@ class MicrophysicsStep:
 
     def close(self):
         self.p3.close()
+
+
+def shoc_postprocess(ctx: Context, qc, qc2, inv_qc_relvar, ncol: int, nlev: int, cldfrac_liq=None,
+                     stream=None):
+    """Enqueue p3x_shoc_postprocess: SHOC's inv_qc_relvar from qc and its variance qc2
+    (dense device arrays), and cldfrac_liq clipped to at most 1 when given."""
+    st = load().p3x_shoc_postprocess(ctx.handle, _addr(qc), _addr(qc2), _addr(cldfrac_liq),
+                                     _addr(inv_qc_relvar), int(ncol), int(nlev), stream)
+    if st != P3X_OK:
+        raise P3Error(st, "p3x_shoc_postprocess")
 
 
 def column_budget(ctx: Context, fields: dict, ncol: int, nlev: int, mass, energy, stream=None,

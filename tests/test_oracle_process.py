@@ -150,3 +150,21 @@ def test_column_budget_closed_forms():
     f["qv"] = z
     m, e = oracle.column_budget(f)
     np.testing.assert_allclose(e, (1004.64 * 250.0 * 72000.0 + 100.0 * 1e5) / 9.80616, rtol=1e-14)
+
+
+def test_shoc_postprocess_closed_form():
+    """SHOC's post-process (eamxx_shoc_process_interface.hpp:288-300): 1 where qc or
+    its variance is zero; qc^2/qc2 inside [0.001, 10]; the clamps outside it; the
+    liquid cloud fraction clipped to at most 1 and otherwise untouched."""
+    qc = np.array([[0.0, 1e-4, 1e-4, 2e-4, 1e-6, 3e-5]])
+    qc2 = np.array([[1e-9, 0.0, 1e-8, 4e-9, 1e-8, 9e-10]])
+    liq = np.array([[0.2, 1.0, 1.5, 0.0, 0.999, 7.0]])
+    r, cl = oracle.shoc_postprocess(qc, qc2, liq)
+    want = np.array([[1.0, 1.0, 1.0, 10.0, 0.001, 1.0]])
+    # 1e-4^2/1e-8 = 1, 2e-4^2/4e-9 = 10 (clamped at 10), 1e-6^2/1e-8 = 1e-4 -> 0.001,
+    # 3e-5^2/9e-10 = 1
+    np.testing.assert_allclose(r, want, rtol=1e-15)
+    mid = oracle.shoc_postprocess(np.array([[3e-4]]), np.array([[2e-8]]))[0]
+    assert mid[0, 0] == 3e-4 * 3e-4 / 2e-8  # inside the bounds: the ratio itself, 4.5
+    np.testing.assert_array_equal(cl, np.array([[0.2, 1.0, 1.0, 0.0, 0.999, 1.0]]))
+    assert oracle.shoc_postprocess(qc, qc2)[1] is None
