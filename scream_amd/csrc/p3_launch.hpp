@@ -41,10 +41,21 @@ inline int sed_levels_per_lane(int nlev, int lpc) {
   return l <= kMaxLevelsPerLane ? kMaxLevelsPerLane : 0;
 }
 
+// A second stream for the liquid group's sedimentation kernel: with one, that
+// kernel runs beside the ice group's (they touch disjoint fields), forked from
+// s after part 2 (event fork) and joined back into s before part 3 (event join).
+struct SedFork {
+  hipStream_t side;
+  hipEvent_t fork, join;
+};
+
 // hipErrorInvalidValue for a shape that is not compiled in.
-// `between` (optional) is recorded on s after part 2, before sedimentation.
-hipError_t launch_p3_fp64(const KParams& p, const Shape& sh, hipStream_t s, hipEvent_t between = nullptr);
-hipError_t launch_p3_mixed(const KParams& p, const Shape& sh, hipStream_t s, hipEvent_t between = nullptr);
+// `between` (optional) is recorded on s after part 2, before sedimentation;
+// `fork` (optional) runs the two sedimentation kernels concurrently.
+hipError_t launch_p3_fp64(const KParams& p, const Shape& sh, hipStream_t s, hipEvent_t between = nullptr,
+                          const SedFork* fork = nullptr);
+hipError_t launch_p3_mixed(const KParams& p, const Shape& sh, hipStream_t s, hipEvent_t between = nullptr,
+                           const SedFork* fork = nullptr);
 
 // ---- batched form (p3_batch.hpp)
 // Sets whose sedimentation kernel uses the same levels-per-lane count.

@@ -63,55 +63,67 @@ struct Geo {
 // k_top + j*L + L-1 from the species' top level k_top down: sedimentation only
 // moves mass down, so the levels above k_top take no part, and spreading only
 // [k_top, nlev) over the lanes keeps them busy (rain, below the freezing
-// level, fills a quarter of a column).  Loads the species' fields in that
-// mapping, runs the substeps and stores the fields; the rain run also stores
-// the rain flux at the interface below each owned level.  prt (every lane)
-// is the mass that left through the surface.
+// level, fills a quarter of a column).  The state of one column's run, held
+// in registers: load() reads the species' fields in that mapping, substep()
+// is one CFL substep, store() writes the fields back (the rain run also
+// stores the rain flux at the interface below each owned level).  Every lane
+// of the group calls each of them together (the CFL maximum, find_bottom and
+// the upwind flux are cross-lane operations inside the group).
 template <int SP, class R, int LPC, int L>
-__device__ __forceinline__ void sediment_run(const KParams& p, int i, int j, int k_top, double& prt,
-                                          int& nsub_out, unsigned& err, const R* __restrict__ ice_tab) {
-  constexpr int NF = SP == 2 ? 4 : 2;  // fields held (cloud always holds nc: its DSD rewrites it)
-  const int nlev = p.nlev, kbot = nlev - 1;
-  const Runtime& rt = p.rt;
-  const int nfield = SP == 2 ? 4 : (SP == 1 ? 2 : (p.predictNc ? 2 : 1));
-  double* fld[NF];
-  if (SP == 0) { fld[0] = p.qc; fld[1] = p.nc; }
-  if (SP == 1) { fld[0] = p.qr; fld[1] = p.nr; }
-  if (SP == 2) { fld[0] = p.qi; fld[1] = p.ni; fld[2] = p.qm; fld[3] = p.bm; }
-  const double* cfp = SP == 0 ? p.cld_frac_l : (SP == 1 ? p.cld_frac_r : p.cld_frac_i);
-  const int coef_f = SP == 0 ? SC_ACN : (SP == 1 ? SC_RHOFACR : SC_RHOFACI);
-  const int kbase = k_top + j * L;
-
+struct SedRun {
+  static constexpr int NF = SP == 2 ? 4 : 2;  // fields held (cloud always holds nc: its DSD rewrites it)
   Geo<L> g;
   double q[NF][L], cf[L], coef[L], cfl_ice[L], pflux[L];
-  int my_bot = -1;
-#pragma unroll
-  for (int t = 0; t < L; ++t) {
-    const int k = kbase + t;
-    const bool ok = k < nlev;
-    const size_t x = p.at(i, ok ? k : 0);
-    const double dz = ok ? p.dz[x] : 1.0;
-    const double dp = ok ? p.dpres[x] : 1.0;
-    g.rho[t] = dp / dz / gravit;
-    g.inv_rho[t] = 1 / g.rho[t];
-    g.inv_dz[t] = 1 / dz;
-#pragma unroll
-    for (int f = 0; f < NF; ++f) q[f][t] = ok ? fld[f][x] : 0.0;
-    cf[t] = ok ? cfp[x] : 1.0;
-    coef[t] = ok ? p.sc(coef_f, i, ok ? k : 0) : 0.0;
-    cfl_ice[t] = (SP == 2 && ok) ? p.cld_frac_l[x] : 1.0;
-    pflux[t] = 0;
-    // find_bottom, P3I/p3_find_impl.hpp:52-92 (find_top gave k_top)
-    if (ok && q[0][t] >= QSMALL) my_bot = k;
-  }
-  int k_bot = grp_max<LPC>(my_bot);
+  int kbase, k_top, k_bot;
+  double dt_left, prt_accum;  // prt_accum: mass through the surface, on the lane owning the lowest level
+  int nsub;
 
-  double dt_left = p.dt, prt_accum = 0;
-  int nsub = 0;
-  // One substep; the first (in-cloud values limited, FIRST) is peeled off the
-  // loop, so that neither carries the other's branch.
-  auto substep = [&](auto first_tag) {
-    constexpr bool first = decltype(first_tag)::value;
+  __device__ __forceinline__ static void fields(const KParams& p, double* (&fld)[NF]) {
+    if constexpr (SP == 0) { fld[0] = p.qc; fld[1] = p.nc; }
+    if constexpr (SP == 1) { fld[0] = p.qr; fld[1] = p.nr; }
+    if constexpr (SP == 2) { fld[0] = p.qi; fld[1] = p.ni; fld[2] = p.qm; fld[3] = p.bm; }
+  }
+
+  __device__ __forceinline__ void load(const KParams& p, int i, int j, int top) {
+    const int nlev = p.nlev;
+    double* fld[NF];
+    fields(p, fld);
+    const double* cfp = SP == 0 ? p.cld_frac_l : (SP == 1 ? p.cld_frac_r : p.cld_frac_i);
+    const int coef_f = SP == 0 ? SC_ACN : (SP == 1 ? SC_RHOFACR : SC_RHOFACI);
+    k_top = top;
+    kbase = k_top + j * L;
+    int my_bot = -1;
+#pragma unroll
+    for (int t = 0; t < L; ++t) {
+      const int k = kbase + t;
+      const bool ok = k < nlev;
+      const size_t x = p.at(i, ok ? k : 0);
+      const double dz = ok ? p.dz[x] : 1.0;
+      const double dp = ok ? p.dpres[x] : 1.0;
+      g.rho[t] = dp / dz / gravit;
+      g.inv_rho[t] = 1 / g.rho[t];
+      g.inv_dz[t] = 1 / dz;
+#pragma unroll
+      for (int f = 0; f < NF; ++f) q[f][t] = ok ? fld[f][x] : 0.0;
+      cf[t] = ok ? cfp[x] : 1.0;
+      coef[t] = ok ? p.sc(coef_f, i, ok ? k : 0) : 0.0;
+      cfl_ice[t] = (SP == 2 && ok) ? p.cld_frac_l[x] : 1.0;
+      pflux[t] = 0;
+      // find_bottom, P3I/p3_find_impl.hpp:52-92 (find_top gave k_top)
+      if (ok && q[0][t] >= QSMALL) my_bot = k;
+    }
+    k_bot = grp_max<LPC>(my_bot);
+    dt_left = p.dt;
+    prt_accum = 0;
+    nsub = 0;
+  }
+
+  // One substep; `first` (the first substep: in-cloud values limited) is a
+  // constant after inlining wherever the caller knows it.
+  __device__ __forceinline__ void substep(const KParams& p, int j, bool first, const R* __restrict__ ice_tab) {
+    const int nlev = p.nlev, kbot = nlev - 1;
+    const Runtime& rt = p.rt;
+    const int nfield = SP == 2 ? 4 : (SP == 1 ? 2 : (p.predictNc ? 2 : 1));
     double V1[L], V2[L];
     double co = 0;
 #pragma unroll
@@ -127,7 +139,7 @@ __device__ __forceinline__ void sediment_run(const KParams& p, int i, int j, int
           const R cfl = R(cf[t]);
           const R qc = R(q[0][t]), nc = R(q[1][t]);
           R qc_incld, nc_incld;
-          if constexpr (first) {
+          if (first) {
             const R icfl = R(1) / cfl;
             if (q[0][t] >= QSMALL) { qc_incld = vmin(qc * icfl, R(incloud_limit)); nc_incld = vmax(nc * icfl, R(0)); }
             else { qc_incld = 0; nc_incld = 0; }
@@ -149,7 +161,7 @@ __device__ __forceinline__ void sediment_run(const KParams& p, int i, int j, int
           const R cfr = R(cf[t]);
           const R qr = R(q[0][t]), nr = R(q[1][t]);
           R qr_incld, nr_incld;
-          if constexpr (first) {
+          if (first) {
             const R icfr = R(1) / cfr;
             if (q[0][t] >= QSMALL) { qr_incld = vmin(qr * icfr, R(precip_limit)); nr_incld = vmax(nr * icfr, R(0)); }
             else { qr_incld = 0; nr_incld = 0; }
@@ -173,7 +185,7 @@ __device__ __forceinline__ void sediment_run(const KParams& p, int i, int j, int
           const R cfi = R(cf[t]);
           const R qi = R(q[0][t]), ni = R(q[1][t]), qm = R(q[2][t]), bm = R(q[3][t]);
           R qi_incld, ni_incld, qm_incld, bm_incld;
-          if constexpr (first) {
+          if (first) {
             const R icfi = R(1) / cfi, icfl = R(1) / R(cfl_ice[t]);
             if (q[0][t] >= QSMALL) { qi_incld = vmin(qi * icfi, R(incloud_limit)); ni_incld = vmax(ni * icfi, R(0)); }
             else { qi_incld = 0; ni_incld = 0; }
@@ -260,32 +272,49 @@ __device__ __forceinline__ void sediment_run(const KParams& p, int i, int j, int
     if (k_bot != kbot) k_bot += 1;
     dt_left -= dt_sub;
     nsub++;
-  };
-  if (dt_left > dt_left_tol) substep(std::true_type{});
-#pragma unroll 1
-  while (dt_left > dt_left_tol) {
-    // The substep count is bounded by the CFL number (a few hundred at most for
-    // physical inputs); the cap only guarantees termination for corrupt inputs.
-    if (nsub >= kMaxSubsteps) { err |= ERR_SUBSTEP_LIMIT; break; }
-    substep(std::false_type{});
   }
+
   // The substeps changed levels k_top .. k_bot only (k_bot has moved down one
   // level after each substep, past the last one's lowest updated level): the
   // levels below keep their loaded values and are not stored again.
+  __device__ __forceinline__ void store(const KParams& p, int i) const {
+    const int nlev = p.nlev;
+    double* fld[NF];
+    fields(p, fld);
 #pragma unroll
-  for (int t = 0; t < L; ++t) {
-    const int k = kbase + t;
-    if (k < nlev) {
-      const size_t x = p.at(i, k);
-      if (k <= k_bot) {
+    for (int t = 0; t < L; ++t) {
+      const int k = kbase + t;
+      if (k < nlev) {
+        const size_t x = p.at(i, k);
+        if (k <= k_bot) {
 #pragma unroll
-        for (int f = 0; f < NF; ++f) fld[f][x] = q[f][t];
+          for (int f = 0; f < NF; ++f) fld[f][x] = q[f][t];
+        }
+        if (SP == 1) p.precip_liq_flux[p.atf(i, k + 1)] = pflux[t];  // rain_sed :140-153, interface k+1
       }
-      if (SP == 1) p.precip_liq_flux[p.atf(i, k + 1)] = pflux[t];  // rain_sed :140-153, interface k+1
     }
   }
-  prt = grp_sumd<LPC>(prt_accum);  // only the lane owning kbot accumulated
-  nsub_out = nsub;
+};
+
+// One species' run over the group's column, whole: load, the substeps (the
+// first peeled off the loop, so that neither carries the other's branch),
+// store.  prt (every lane) is the mass that left through the surface.
+template <int SP, class R, int LPC, int L>
+__device__ __forceinline__ void sediment_run(const KParams& p, int i, int j, int k_top, double& prt,
+                                          int& nsub_out, unsigned& err, const R* __restrict__ ice_tab) {
+  SedRun<SP, R, LPC, L> r;
+  r.load(p, i, j, k_top);
+  if (r.dt_left > dt_left_tol) r.substep(p, j, true, ice_tab);
+#pragma unroll 1
+  while (r.dt_left > dt_left_tol) {
+    // The substep count is bounded by the CFL number (a few hundred at most for
+    // physical inputs); the cap only guarantees termination for corrupt inputs.
+    if (r.nsub >= kMaxSubsteps) { err |= ERR_SUBSTEP_LIMIT; break; }
+    r.substep(p, j, false, ice_tab);
+  }
+  r.store(p, i);
+  prt = grp_sumd<LPC>(r.prt_accum);  // only the lane owning kbot accumulated
+  nsub_out = r.nsub;
 }
 
 // find_top (P3I/p3_find_impl.hpp:14-50) of NS species (mixing ratios f[s]) over
@@ -497,20 +526,25 @@ __device__ __forceinline__ void part3_level(const KParams& p, int k, int i, bool
 // kernel holds only its own group's registers and field pointers.
 enum SedGroup { SED_LIQ = 0, SED_ICE = 1 };
 
-// One block's work for group G: WPB wavefronts of 64/LPC columns each, the
-// CB = WPB * 64/LPC columns starting at column bid * CB of the column set p,
-// with LPC lanes per column (part 3 follows in p3_part3_lvl_kernel).
-template <class R, int LPC, int L, int WPB, int G>
-__device__ __forceinline__ void sed_block(const KParams& p, int bid, const R* __restrict__ ice_tab) {
-  constexpr int CPW = 64 / LPC;  // columns per wavefront
-  constexpr int CB = WPB * CPW;  // columns per block
+// The ice group's per-column outputs once column i's run has ended (any one
+// lane of the group writes them): the surface precipitation, its post-amble
+// accumulation (eamxx_p3_process_interface.hpp:245-258) and the substep count.
+__device__ __forceinline__ void ice_column_out(const KParams& p, int i, bool ran, double prt, int nsub) {
+  double ice = 0;
+  if (ran) ice += prt * INV_RHO_H2O * (1 / p.dt);
+  p.precip_ice_surf[i] = ice;
+  if (p.amble.on) p.amble.precip_ice_mass[i] += ice * RHO_H2O * p.dt;
+  if (p.dbg_substeps) p.dbg_substeps[(size_t)2 * p.dbg_stride + i] = nsub;
+}
+
+// One wavefront's work for group G: the 64/LPC columns starting at column
+// col0, with LPC lanes per column.
+template <class R, int LPC, int L, int G>
+__device__ __forceinline__ void sed_wave(const KParams& p, int col0, const R* __restrict__ ice_tab) {
   static_assert(64 % LPC == 0, "a column group must divide the wavefront");
-  static_assert((CB & (CB - 1)) == 0, "columns per block must be a power of two");
   const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
   const int j = lane % LPC;
-  const int col0 = bid * CB;
-  const int i = col0 + wave * CPW + lane / LPC;
+  const int i = col0 + lane / LPC;
   const int nlev = p.nlev, kbot = nlev - 1;
   const int kbase = j * L;
   unsigned err = 0;
@@ -562,19 +596,21 @@ __device__ __forceinline__ void sed_block(const KParams& p, int bid, const R* __
     find_tops<LPC, L, 1>(p, i, j, run, {p.qi}, top);
     const int top_i = top[0];
     sediment<2, R, LPC, L>(p, i, j, top_i, prt_i, ns_i, err, ice_tab);
-    if (live) {
-#pragma unroll
-      for (int t = 0; t < L; ++t) {
-        if (kbase + t != kbot) continue;
-        double ice = 0;
-        if (top_i < nlev) ice += prt_i * INV_RHO_H2O * inv_dt;
-        p.precip_ice_surf[i] = ice;
-        if (p.amble.on) p.amble.precip_ice_mass[i] += ice * RHO_H2O * p.dt;
-      }
-      if (j == 0 && p.dbg_substeps) p.dbg_substeps[(size_t)2 * p.dbg_stride + i] = ns_i;
-    }
+    if (live && j == 0) ice_column_out(p, i, top_i < nlev, prt_i, ns_i);
   }
   if (err) atomicOr(p.err, err);
+}
+
+// One block's work for group G: WPB wavefronts of 64/LPC columns each, the
+// CB = WPB * 64/LPC columns starting at column bid * CB of the column set p,
+// with LPC lanes per column (part 3 follows in p3_part3_lvl_kernel).
+template <class R, int LPC, int L, int WPB, int G>
+__device__ __forceinline__ void sed_block(const KParams& p, int bid, const R* __restrict__ ice_tab) {
+  constexpr int CPW = 64 / LPC;  // columns per wavefront
+  constexpr int CB = WPB * CPW;  // columns per block
+  static_assert((CB & (CB - 1)) == 0, "columns per block must be a power of two");
+  const int wave = threadIdx.x >> 6;
+  sed_wave<R, LPC, L, G>(p, bid * CB + wave * CPW, ice_tab);
 }
 
 // Part 3 as a kernel of its own, one thread per (level, column) on the
@@ -615,8 +651,8 @@ template <class R, int G, int L> constexpr int sed_waves_per_simd() {
 // block's lanes read a few columns of many levels, so neighbouring blocks share
 // every cache line they touch: give each XCD a contiguous range of blocks, so
 // that the blocks sharing lines run on one XCD at about the same time.
+constexpr int kXcds = 8;
 __device__ __forceinline__ int xcd_block(int b, int nb) {
-  constexpr int kXcds = 8;
   const int q = nb / kXcds, r = nb % kXcds, x = b % kXcds, n = b / kXcds;
   return x * q + (x < r ? x : r) + n;
 }

@@ -310,6 +310,9 @@ extern "C" int p3x_init(const p3x_table_data* t, p3x_context** out) {
   if (e == hipSuccess) e = hipEventCreate(&c->ev0);
   if (e == hipSuccess) e = hipEventCreate(&c->ev1);
   if (e == hipSuccess) e = hipEventCreate(&c->ev_mid);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming);
   if (e == hipSuccess) e = hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, c->device);
   if (e != hipSuccess) {
     p3x_finalize(c);
@@ -336,6 +339,9 @@ extern "C" void p3x_finalize(p3x_context* c) {
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->ev_mid) (void)hipEventDestroy(c->ev_mid);
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+  if (c->side) (void)hipStreamDestroy(c->side);
   delete c;
 }
 
@@ -444,7 +450,10 @@ int p3x_detail::main_impl(p3x_context* c, const p3x_runtime* runtime, const p3x_
   }
   e = hipEventRecord(c->ev0, s);
   if (e != hipSuccess) return P3X_ERR_DEVICE;
-  e = o.precision == P3X_FP64 ? p3d::launch_p3_fp64(k, sh, s, c->ev_mid) : p3d::launch_p3_mixed(k, sh, s, c->ev_mid);
+  // the liquid sedimentation kernel beside the ice kernel (21600 x 72 FP64: 1255 -> 1216 us per step)
+  const p3d::SedFork fork{c->side, c->ev_fork, c->ev_join};
+  e = o.precision == P3X_FP64 ? p3d::launch_p3_fp64(k, sh, s, c->ev_mid, &fork)
+                              : p3d::launch_p3_mixed(k, sh, s, c->ev_mid, &fork);
   if (e == hipErrorInvalidValue) return P3X_ERR_ARG;
   if (e != hipSuccess) return P3X_ERR_DEVICE;
   e = hipEventRecord(c->ev1, s);

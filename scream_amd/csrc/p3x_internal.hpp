@@ -20,6 +20,10 @@ struct p3x_context {
   unsigned* d_err = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipEvent_t ev_mid = nullptr;  // between the part 1 + 2 and the sedimentation kernels
+  // the liquid sedimentation kernel's stream (beside the ice kernel) and its
+  // fork and join events
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   hipStream_t last_stream = nullptr;
   // the stream of the last p3x_main on this context (the last user of d_scratch);
   // ev1 marks the end of its work once main_launched is set
