@@ -572,8 +572,9 @@ def summarize_pmc(fetch_dir, write_dir, out_path=None):
                           "bytes": sum(f.values()) * scale_f + sum(w.values()) * scale_w,
                           "kernels": sorted(f)}
     out = {"g64_fp64": {
-        "workload": "p3_main FP64 drop-in, 21600x72, dt=1800 s (bench.py --no-cpu --no-forms "
-                    "--no-process --batch-sets 0 --serve-seconds 0)",
+        "workload": "p3_main FP64 drop-in, 21600x72, dt=1800 s (bench.py --steps 5 --warmup 2 --no-cpu "
+                    "--no-forms --no-process --batch-sets 0 --serve-seconds 0 --no-table-gen "
+                    "--no-column-major)",
         "copy_probe": {"bytes_each_way": COPY_PROBE_BYTES, "FETCH_SIZE_KiB": copy_f,
                        "WRITE_SIZE_KiB": copy_w, "read_bytes_per_KiB": scale_f,
                        "write_bytes_per_KiB": scale_w},
