@@ -23,32 +23,62 @@
 
 namespace p3d {
 
+// Reductions over aligned groups of LPC lanes, all lanes getting the result.
+// Step o combines each lane with a lane of the other half of its 2o-lane
+// block; before the step both halves are uniform, so any pairing of the halves
+// does: inside a 16-lane row by DPP (o = 1, 2: quad permutes; 4, 8: the
+// half-row and row mirrors), across rows by ds_swizzle (o = 16, xor inside 32
+// lanes) and across the wavefront's halves by ds_bpermute (o = 32).  A lane
+// reads only lanes of its own group, and every lane of a group must be active
+// (a DPP read from an inactive lane returns the reader's own value): the
+// sedimentation code calls these with whole groups, never with part of one.
+template <int O>
+__device__ __forceinline__ int partner(int v) {
+  if constexpr (O == 1) return __builtin_amdgcn_update_dpp(v, v, 0xB1, 0xF, 0xF, false);  // quad_perm 1,0,3,2
+  else if constexpr (O == 2) return __builtin_amdgcn_update_dpp(v, v, 0x4E, 0xF, 0xF, false);  // quad_perm 2,3,0,1
+  else if constexpr (O == 4) return __builtin_amdgcn_update_dpp(v, v, 0x141, 0xF, 0xF, false);  // row_half_mirror
+  else if constexpr (O == 8) return __builtin_amdgcn_update_dpp(v, v, 0x140, 0xF, 0xF, false);  // row_mirror
+  else if constexpr (O == 16) return __builtin_amdgcn_ds_swizzle(v, 0x401F);  // lane ^ 16 within 32
+  else return __shfl_xor(v, 32, 64);
+}
+template <int O>
+__device__ __forceinline__ double partner(double v) {
+  return __hiloint2double(partner<O>(__double2hiint(v)), partner<O>(__double2loint(v)));
+}
+
 template <int LPC>
 __device__ __forceinline__ int grp_min(int v) {
-#pragma unroll
-  for (int o = LPC / 2; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+  if constexpr (LPC > 1) {
+    v = grp_min<LPC / 2>(v);
+    v = min(v, partner<LPC / 2>(v));
+  }
   return v;
 }
 template <int LPC>
 __device__ __forceinline__ int grp_max(int v) {
-#pragma unroll
-  for (int o = LPC / 2; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+  if constexpr (LPC > 1) {
+    v = grp_max<LPC / 2>(v);
+    v = max(v, partner<LPC / 2>(v));
+  }
   return v;
 }
 template <int LPC>
 __device__ __forceinline__ double grp_maxd(double v) {
-#pragma unroll
-  for (int o = LPC / 2; o > 0; o >>= 1) {
-    const double w = __shfl_xor(v, o, 64);
+  if constexpr (LPC > 1) {
+    v = grp_maxd<LPC / 2>(v);
+    const double w = partner<LPC / 2>(v);
     v = w > v ? w : v;
   }
   return v;
 }
-
+// (Used where one lane of the group holds the value and the others 0, so the
+// order of the additions does not matter.)
 template <int LPC>
 __device__ __forceinline__ double grp_sumd(double v) {
-#pragma unroll
-  for (int o = LPC / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if constexpr (LPC > 1) {
+    v = grp_sumd<LPC / 2>(v);
+    v += partner<LPC / 2>(v);
+  }
   return v;
 }
 
@@ -234,25 +264,24 @@ struct SedRun {
     const int tmpint1 = (int)(Co_max + 1);
     const double dt_sub = dt_left / tmpint1;
     const int k_temp = (k_bot == kbot) ? k_bot : k_bot + 1;
-    // fluxes F = V r rho on [k_top, k_temp] (V = 0 under the window)
+    // fluxes F = V r rho.  Only the levels in [k_top, k_temp] update, and
+    // they read F of their own level and of the level above, which is in
+    // that range too (or is k_top's upper boundary, flux 0): F elsewhere is
+    // never read, so it is not masked.
     double F[NF][L];
 #pragma unroll
     for (int t = 0; t < L; ++t) {
-      const int k = kbase + t;
-      const bool in = k >= k_top && k <= k_temp;
 #pragma unroll
       for (int f = 0; f < NF; ++f) {
         const double V = (SP == 2) ? (f == 1 ? V2[t] : V1[t]) : (f == 0 ? V1[t] : V2[t]);
-        F[f][t] = in ? V * q[f][t] * g.rho[t] : 0.0;
+        F[f][t] = V * q[f][t] * g.rho[t];
       }
     }
-    // flux out of the last level of the lane above
+    // flux out of the last level of the lane above (lane 0's first level is
+    // k_top, whose incoming flux is 0)
     double Fin[NF];
 #pragma unroll
-    for (int f = 0; f < NF; ++f) {
-      Fin[f] = __shfl_up(F[f][L - 1], 1, LPC);
-      if (j == 0) Fin[f] = 0;
-    }
+    for (int f = 0; f < NF; ++f) Fin[f] = __shfl_up(F[f][L - 1], 1, LPC);
 #pragma unroll
     for (int t = 0; t < L; ++t) {
       const int k = kbase + t;
@@ -260,7 +289,8 @@ struct SedRun {
 #pragma unroll
         for (int f = 0; f < NF; ++f) {
           if (f < nfield) {
-            const double Fa = (k == k_top) ? 0.0 : (t == 0 ? Fin[f] : F[f][t - 1]);
+            // level kbase + t is k_top exactly when t = 0 and j = 0
+            const double Fa = t == 0 ? (j == 0 ? 0.0 : Fin[f]) : F[f][t > 0 ? t - 1 : 0];
             const double fluxdiv = (Fa - F[f][t]) * g.inv_dz[t];
             q[f][t] = q[f][t] + fluxdiv * dt_sub * g.inv_rho[t];
           }
