@@ -277,6 +277,7 @@ typedef struct p3x_batch_options {
   int part12_waves_per_simd; /* 0 = default, 1 or 2, as in p3x_launch                    */
   int blocks_per_cu;         /* persistent grid of each launch: this many blocks per CU;  */
                              /* 0 = as many as fit (occupancy); -1 = one block per unit   */
+                             /* (the default: the kernels then run without their loop)   */
   int use_graph;             /* 1 = capture the launches into a HIP graph once, replay it */
 } p3x_batch_options;
 

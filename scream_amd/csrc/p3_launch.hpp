@@ -73,6 +73,7 @@ struct BatchPlan {
   const int* first2;    // device array, nsets + 1 entries: pointwise tiles (parts 1 to 3)
   int nsets;
   int grid, grid2;      // persistent blocks of the part 1 block kernel and the pointwise launches
+  int total2;           // pointwise virtual blocks (first2[nsets]); grid2 >= total2: one block each, no loop
   int nclass;
   SedClass cls[kMaxLevelsPerLane];
 };

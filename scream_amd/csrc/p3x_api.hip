@@ -637,6 +637,7 @@ int build_plan(p3x_batch* b) {
     if (ci == 0) {
       b->plan.grid = grid_for(c->ncu, b->opt.blocks_per_cu, lay.first[n], r1);
       b->plan.grid2 = grid_for(c->ncu, b->opt.blocks_per_cu, lay.first2[n], r2);
+      b->plan.total2 = lay.first2[n];
     }
   }
 
@@ -676,6 +677,7 @@ extern "C" void p3x_batch_defaults(p3x_batch_options* o) {
   if (!o) return;
   std::memset(o, 0, sizeof(*o));
   o->precision = P3X_MIXED;
+  o->blocks_per_cu = -1;
   o->use_graph = 1;
 }
 

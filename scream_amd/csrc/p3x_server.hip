@@ -160,8 +160,11 @@ int launch_slot(p3x_server* srv, Slot& sl) {
   plan.first = (const int*)(sl.d_desc + o_first);
   plan.first2 = (const int*)(sl.d_desc + o_first2);
   plan.nsets = n;
-  plan.grid = grid_for(srv->ctx->ncu, 0, lay.first[n], srv->occ1);
-  plan.grid2 = grid_for(srv->ctx->ncu, 0, lay.first2[n], srv->occ2);
+  // one block per virtual block: the pointwise kernels then run without their
+  // persistent loop, which costs them registers
+  plan.grid = grid_for(srv->ctx->ncu, -1, lay.first[n], srv->occ1);
+  plan.grid2 = grid_for(srv->ctx->ncu, -1, lay.first2[n], srv->occ2);
+  plan.total2 = lay.first2[n];
   plan.nclass = (int)lay.cls.size();
   for (int c = 0; c < plan.nclass; ++c) {
     const Layout::Cls& lc = lay.cls[c];
@@ -170,7 +173,7 @@ int launch_slot(p3x_server* srv, Slot& sl) {
     sc.sets = (const p3d::KParams*)(sl.d_desc + o_sed) + lc.set_off;
     sc.first = (const int*)(sl.d_desc + o_sfirst) + lc.first_off;
     sc.nsets = lc.nsets;
-    sc.grid = grid_for(srv->ctx->ncu, 0, lc.total, srv->occ_sed[lc.L]);
+    sc.grid = grid_for(srv->ctx->ncu, -1, lc.total, srv->occ_sed[lc.L]);
   }
   // the neighbour stages of process members before their steps ...
   for (int i = 0; i < n; ++i) {
