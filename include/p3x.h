@@ -158,8 +158,9 @@ typedef struct p3x_launch {
    * default, else one of 16x128, 16x256, 32x256, 64x256, 32x512, 64x512      */
   int columns_per_block;
   int threads_per_block;
-  /* sedimentation + part 3 kernel: lanes per column, 0 for the default, else
-   * 16, 32 or 64; nk may not exceed 8 x this (nk <= 512 in any case)         */
+  /* sedimentation kernels: lanes per column, 0 for each species group's
+   * default, else 16, 32 or 64 for both groups; nk may not exceed 8 x this
+   * (nk <= 512 in any case)                                                   */
   int sed_lanes_per_column;
   /* part 2 kernel: wavefronts per SIMD its registers must allow, 0 for the
    * default (2), else 1 or 2; an explicit block shape without it means 1    */
