@@ -82,8 +82,9 @@ void default_shape(int precision, int nk, p3d::Shape* sh) {
   // (at most 5 levels per lane keep two wavefronts per SIMD; the batched form
   // runs both groups on lpc, which its tuner also searches).  Since the group
   // reductions run on DPP inside a 16-lane row, 16 lanes per column pay off for
-  // the FP64 ice group at up to 80 levels: whole step at 21600 x 72, three runs
-  // each, 1110 us against 1170 at 32 lanes (mixed: no difference).
+  // the FP64 ice group at up to 80 levels: whole step at 21600 x 72, four runs
+  // of each build alternating, 1143 us against 1159 at 32 lanes (mixed: no
+  // difference; at 128 levels 32 or 64 lanes for both groups were slower).
   sh->lpc = nk <= 80 ? 16 : (nk <= 32 * 5 ? 32 : 64);
   if (precision == P3X_MIXED) sh->lpc_ice = nk <= 32 * 5 ? 32 : 64;
   else sh->lpc_ice = nk <= 80 ? 16 : (nk <= 96 ? 32 : 64);
