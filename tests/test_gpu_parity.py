@@ -82,6 +82,23 @@ def test_fp64_matches_oracle(ctx, oracle_tables, ncol, nlev, dt, pnc, pccn, seed
     assert np.all(np.abs(after - before) <= BUDGET_RTOL * np.abs(before))
 
 
+# Level counts on both sides of every change of the default lanes per column and
+# levels per lane (16 lanes up to 80 levels, 32 up to 96 or 160, then 64; at most
+# 5 levels per lane before the 6- and 8-level instantiations).
+DEFAULT_SHAPE_LEVELS = [1, 2, 16, 17, 33, 49, 65, 80, 81, 96, 97, 160, 161, 200, 300]
+
+
+@pytest.mark.parametrize("nlev", DEFAULT_SHAPE_LEVELS)
+def test_fp64_default_shapes_across_level_counts(ctx, oracle_tables, nlev):
+    """The default launch shape at each level count (its lanes per column and levels
+    per lane for both sedimentation groups) matches the oracle."""
+    inp = make_columns(70, nlev, seed=100 + nlev, dry_every=7, evap_every=9)
+    want, wdbg, nerr = oracle.run_oracle(inp, oracle_tables, 600.0, True, False, math=MATH)
+    assert nerr == 0
+    got, gdbg, _ = run_gpu(ctx, inp, 600.0, True, False)
+    _check_fp64(got, gdbg, want, wdbg, f"70x{nlev} default shape")
+
+
 def test_fp64_nonzero_history_inputs(ctx, oracle_tables):
     """The exchange (history) fields are in/out: p3_main adds to what the caller holds."""
     inp = make_columns(128, 72, seed=17, dry_every=5, evap_every=6)
