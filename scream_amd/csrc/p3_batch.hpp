@@ -1,11 +1,12 @@
-// p3_batch.hpp -- persistent kernels that run one P3 step of many independent
-// column sets (each with its own ncol, nlev, dt and flags) in one launch each.
+// p3_batch.hpp -- kernels that run one P3 step of many independent column
+// sets (each with its own ncol, nlev, dt and flags) in one launch each.
 //
 // A set's work is cut into the same blocks its single-set launch would use
 // ("virtual blocks").  first[s] is the index of set s's first virtual block in
-// the concatenation of all sets (first[nsets] = total).  A persistent grid of
-// gridDim.x blocks walks the virtual blocks round-robin; a block finds the set
-// of its virtual block by binary search over first[].
+// the concatenation of all sets (first[nsets] = total).  The grid has one block
+// per virtual block, or (a persistent grid) gridDim.x blocks walk the virtual
+// blocks round-robin; a block finds the set of its virtual block by binary
+// search over first[].
 //
 // The set descriptors are read through the constant address space: the
 // compiler then loads them with scalar loads and knows that the field pointers
@@ -100,7 +101,7 @@ __global__ __launch_bounds__(kTileThreads) void p3_part3_lvl_batch(const KParams
 
 // Sedimentation of species group G (SED_LIQ or SED_ICE).  The sets of a batch
 // share their context's tables, so the ice group stages the first set's ice
-// records in LDS once per persistent block.
+// records in LDS once per block.
 template <class R, int LPC, int L, int WPB, int G>
 __global__ __launch_bounds__(64 * WPB, (sed_waves_per_simd<R, G, L>())) void p3_sed_batch(const KParams* __restrict__ sets,
                                                          const int* __restrict__ first, int nsets) {

@@ -64,7 +64,7 @@ struct SedClass {
   const KParams* sets;  // device array
   const int* first;     // device array, nsets + 1 entries
   int nsets;
-  int grid;             // persistent blocks
+  int grid;             // blocks (one per virtual block, or a persistent grid)
 };
 struct BatchPlan {
   Shape shape;
@@ -72,7 +72,7 @@ struct BatchPlan {
   const int* first;     // device array, nsets + 1 entries: part 1 blocks
   const int* first2;    // device array, nsets + 1 entries: pointwise tiles (parts 1 to 3)
   int nsets;
-  int grid, grid2;      // persistent blocks of the part 1 block kernel and the pointwise launches
+  int grid, grid2;      // blocks of the part 1 block kernel and of the pointwise launches
   int total2;           // pointwise virtual blocks (first2[nsets]); grid2 >= total2: one block each, no loop
   int nclass;
   SedClass cls[kMaxLevelsPerLane];

@@ -588,7 +588,7 @@ void free_plan(p3x_batch* b) {
 hipError_t launch_batch(const p3x_batch* b, hipStream_t s) { return launch_plan(b->opt.precision, b->plan, s); }
 
 // Lay the sets out for b->opt: virtual-block offsets of both launches, the
-// sedimentation classes, persistent grid sizes and (use_graph) the graph.
+// sedimentation classes, grid sizes and (use_graph) the graph.
 int build_plan(p3x_batch* b) {
   p3x_context* c = b->ctx;
   const int n = (int)b->sets.size();
