@@ -108,9 +108,10 @@ __global__ __launch_bounds__(64 * WPB, (sed_waves_per_simd<R, G, L>())) void p3_
   const auto sets4 = (const __attribute__((address_space(4))) KParams*)sets;
   const int total = first[nsets];
   if constexpr (G == SED_ICE) {
-    __shared__ __align__(16) R s_tab[kIceSedTableLen];
+    static_assert(WPB == 4 || !ice_rec_dif<R, L>(), "the LDS budget assumes blocks of 4 wavefronts");
+    __shared__ __align__(16) R s_tab[ice_lds_len<R, L>()];
     if ((int)blockIdx.x >= total) return;  // the whole block leaves before the barrier
-    stage_ice_table<R>(TabOf<R>::get(((const KParams*)sets4)->tab).ice_sed, s_tab);
+    stage_ice_table<R, ice_lds_len<R, L>()>(ice_lds_source<R, L>(*(const KParams*)sets4), s_tab);
     for (int vb = blockIdx.x; vb < total; vb += gridDim.x) {
       const int s = find_set(first, nsets, vb);
       sed_block<R, LPC, L, WPB, G>(*(const KParams*)(sets4 + s), vb - first[s], s_tab);

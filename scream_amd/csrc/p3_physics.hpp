@@ -95,6 +95,8 @@ template <class T> struct TableSet {
   const T* revap;    // [300][10]
   const T* ice_p2;   // [5][4][50][8]: ice quantities 1,2,3,4,6,7,9 and a pad (part 2)
   const T* ice_sed;  // [5][4][50][4]: quantities 0,1,6,7 (ice sedimentation)
+  const T* ice_sed_d;  // [5][4][50][8]: the same 4 quantities, then each one's difference
+                       // to the next size cell (0 in the last; ice sedimentation)
   const T* ice_p3;   // [5][4][50][4]: quantities 5,6,7,11 (part 3)
   const T* rain_v;   // [300][10][2]: vm, vn (rain sedimentation)
 };
@@ -226,6 +228,41 @@ __device__ __forceinline__ void apply_table_ice_rec(const R* __restrict__ tab, c
     const R tmp1 = iproc1 + w4 * (gproc1 - iproc1);
     iproc1 = c[4][q] + w1 * (c[5][q] - c[4][q]);
     gproc1 = c[6][q] + w1 * (c[7][q] - c[6][q]);
+    const R tmp2 = iproc1 + w4 * (gproc1 - iproc1);
+    out[q] = tmp1 + w5 * (tmp2 - tmp1);
+  }
+}
+
+// apply_table_ice_rec for records that also hold, DOFF values after each
+// quantity, its difference to the same quantity of the next size cell,
+// computed on the host in R.  c + w1 * d then has the bits of
+// c + w1 * (c_next - c), so the four size-direction interpolations need no
+// subtraction, and each corner pair is still two 16-byte reads (the value and
+// its difference instead of the two cells).
+template <int NQ, int NR, int DOFF, class R>
+__device__ __forceinline__ void apply_table_ice_dif(const R* __restrict__ tab, const TabIce<R>& t, R (&out)[NQ],
+                                                    int q0 = 0) {
+  const int jj = t.dumjj, ii = t.dumii, i = t.dumi;
+  const R w1 = (t.dum1 - R(i) - R(1));
+  const R w4 = (t.dum4 - R(ii) - R(1));
+  const R w5 = (t.dum5 - R(jj) - R(1));
+  auto rec = [&](int a, int b) { return tab + ((a * rimsize + b) * isize + i) * NR + q0; };
+  R c[4][NQ], d[4][NQ];
+  load_rec<NQ>(rec(jj, ii), c[0]);
+  load_rec<NQ>(rec(jj, ii) + DOFF, d[0]);
+  load_rec<NQ>(rec(jj, ii + 1), c[1]);
+  load_rec<NQ>(rec(jj, ii + 1) + DOFF, d[1]);
+  load_rec<NQ>(rec(jj + 1, ii), c[2]);
+  load_rec<NQ>(rec(jj + 1, ii) + DOFF, d[2]);
+  load_rec<NQ>(rec(jj + 1, ii + 1), c[3]);
+  load_rec<NQ>(rec(jj + 1, ii + 1) + DOFF, d[3]);
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    R iproc1 = c[0][q] + w1 * d[0][q];
+    R gproc1 = c[1][q] + w1 * d[1][q];
+    const R tmp1 = iproc1 + w4 * (gproc1 - iproc1);
+    iproc1 = c[2][q] + w1 * d[2][q];
+    gproc1 = c[3][q] + w1 * d[3][q];
     const R tmp2 = iproc1 + w4 * (gproc1 - iproc1);
     out[q] = tmp1 + w5 * (tmp2 - tmp1);
   }

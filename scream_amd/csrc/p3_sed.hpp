@@ -82,6 +82,34 @@ __device__ __forceinline__ double grp_sumd(double v) {
   return v;
 }
 
+// Sedimentation runs as two kernels over the same column blocks, one per
+// species group: the liquid group (cloud, then rain, with the rain flux and
+// the liquid surface precipitation) and the ice group.  The groups touch
+// disjoint state fields, so their order does not change a bit; apart, each
+// kernel holds only its own group's registers and field pointers.
+enum SedGroup { SED_LIQ = 0, SED_ICE = 1 };
+
+// Wavefronts per SIMD the sedimentation kernels are compiled for: up to 5
+// levels per lane two fit (256 registers); the FP64 ice group at up to 3
+// levels per lane runs at three (168 registers and a 76-byte spill at 3 levels:
+// 736 -> 714 us at 21600 x 72; the FP32 ice group holds 127 registers anyway).
+template <class R, int G, int L> constexpr int sed_waves_per_simd() {
+  return (G == SED_ICE && sizeof(R) == 8 && L <= 3) ? 3 : (L <= 5 ? 2 : 1);
+}
+
+// The ice group interpolates its 4 table quantities from an LDS copy of the
+// ice-sedimentation records.  Where the blocks the kernel's occupancy puts on
+// one CU (blocks of 4 wavefronts) can each hold it in the CU's 160 KB of LDS,
+// the copy is of the records that also carry each quantity's difference to
+// the next size cell (64 KB in FP64, 32 KB in FP32), which saves the four
+// size-direction subtractions per quantity; elsewhere (the FP64 kernels at
+// three wavefronts per SIMD) it is of the plain records (32 KB).
+constexpr int kIceCells = densize * rimsize * isize;
+template <class R, int L> constexpr bool ice_rec_dif() {
+  return sed_waves_per_simd<R, SED_ICE, L>() * kIceCells * 8 * (int)sizeof(R) <= 160 * 1024;
+}
+template <class R, int L> constexpr int ice_lds_len() { return kIceCells * (ice_rec_dif<R, L>() ? 8 : 4); }
+
 // Per-lane geometry of the owned levels.
 template <int L>
 struct Geo {
@@ -98,8 +126,10 @@ struct Geo {
 // is one CFL substep, store() writes the fields back (the rain run also
 // stores the rain flux at the interface below each owned level).  Every lane
 // of the group calls each of them together (the CFL maximum, find_bottom and
-// the upwind flux are cross-lane operations inside the group).
-template <int SP, class R, int LPC, int L>
+// the upwind flux are cross-lane operations inside the group).  DIF: the ice
+// records in ice_tab carry their size-direction differences (the layout the
+// kernel staged, chosen by its own levels per lane, not by this run's L).
+template <int SP, class R, int LPC, int L, bool DIF>
 struct SedRun {
   static constexpr int NF = SP == 2 ? 4 : 2;  // fields held (cloud always holds nc: its DSD rewrites it)
   Geo<L> g;
@@ -238,11 +268,17 @@ struct SedRun {
             // quantities 0, 1, 6, 7 of the records (in FP64 as two halves, which bounds the registers)
             R v[4];
             if constexpr (sizeof(R) == 4) {
-              apply_table_ice_rec<4, 4, R>(ice_tab, ti, v, 0);
+              if constexpr (DIF) apply_table_ice_dif<4, 8, 4, R>(ice_tab, ti, v, 0);
+              else apply_table_ice_rec<4, 4, R>(ice_tab, ti, v, 0);
             } else {
               R a[2], b[2];
-              apply_table_ice_rec<2, 4, R>(ice_tab, ti, a, 0);
-              apply_table_ice_rec<2, 4, R>(ice_tab, ti, b, 2);
+              if constexpr (DIF) {
+                apply_table_ice_dif<2, 8, 4, R>(ice_tab, ti, a, 0);
+                apply_table_ice_dif<2, 8, 4, R>(ice_tab, ti, b, 2);
+              } else {
+                apply_table_ice_rec<2, 4, R>(ice_tab, ti, a, 0);
+                apply_table_ice_rec<2, 4, R>(ice_tab, ti, b, 2);
+              }
               v[0] = a[0]; v[1] = a[1]; v[2] = b[0]; v[3] = b[1];
             }
             const R t_ni_fall = v[0], t_qi_fall = v[1], t_lammax = v[2], t_lammin = v[3];
@@ -329,10 +365,10 @@ struct SedRun {
 // One species' run over the group's column, whole: load, the substeps (the
 // first peeled off the loop, so that neither carries the other's branch),
 // store.  prt (every lane) is the mass that left through the surface.
-template <int SP, class R, int LPC, int L>
+template <int SP, class R, int LPC, int L, bool DIF>
 __device__ __forceinline__ void sediment_run(const KParams& p, int i, int j, int k_top, double& prt,
                                           int& nsub_out, unsigned& err, const R* __restrict__ ice_tab) {
-  SedRun<SP, R, LPC, L> r;
+  SedRun<SP, R, LPC, L, DIF> r;
   r.load(p, i, j, k_top);
   if (r.dt_left > dt_left_tol) r.substep(p, j, true, ice_tab);
 #pragma unroll 1
@@ -394,9 +430,11 @@ __device__ __forceinline__ void sediment(const KParams& p, int i, int j, int k_t
 #pragma unroll
   for (int o = 32; o >= LPC; o >>= 1) lw = max(lw, __shfl_xor(lw, o, 64));
   if (k_top >= nlev) return;
-  if (lw <= 1) sediment_run<SP, R, LPC, 1>(p, i, j, k_top, prt, nsub_out, err, ice_tab);
-  else if (L > 2 && lw <= 2) sediment_run<SP, R, LPC, (L > 2 ? 2 : L)>(p, i, j, k_top, prt, nsub_out, err, ice_tab);
-  else sediment_run<SP, R, LPC, L>(p, i, j, k_top, prt, nsub_out, err, ice_tab);
+  // the ice records' layout is the one the kernel staged for its own L
+  constexpr bool dif = ice_rec_dif<R, L>();
+  if (lw <= 1) sediment_run<SP, R, LPC, 1, dif>(p, i, j, k_top, prt, nsub_out, err, ice_tab);
+  else if (L > 2 && lw <= 2) sediment_run<SP, R, LPC, (L > 2 ? 2 : L), dif>(p, i, j, k_top, prt, nsub_out, err, ice_tab);
+  else sediment_run<SP, R, LPC, L, dif>(p, i, j, k_top, prt, nsub_out, err, ice_tab);
 }
 
 // homogeneous_freezing (P3I/p3_ice_sed_impl.hpp:207-270) + p3_main_part3
@@ -549,13 +587,6 @@ __device__ __forceinline__ void part3_level(const KParams& p, int k, int i, bool
   p.rho_qi[x] = rho_qi;
 }
 
-// Sedimentation runs as two kernels over the same column blocks, one per
-// species group: the liquid group (cloud, then rain, with the rain flux and
-// the liquid surface precipitation) and the ice group.  The groups touch
-// disjoint state fields, so their order does not change a bit; apart, each
-// kernel holds only its own group's registers and field pointers.
-enum SedGroup { SED_LIQ = 0, SED_ICE = 1 };
-
 // The ice group's per-column outputs once column i's run has ended (any one
 // lane of the group writes them): the surface precipitation, its post-amble
 // accumulation (eamxx_p3_process_interface.hpp:245-258) and the substep count.
@@ -669,14 +700,6 @@ __global__ __launch_bounds__(kTileThreads) void p3_part3_lvl_kernel(const KParam
   part3_tile<R>(p, blockIdx.x);
 }
 
-// Wavefronts per SIMD the sedimentation kernels are compiled for: up to 5
-// levels per lane two fit (256 registers); the FP64 ice group at up to 3
-// levels per lane runs at three (168 registers and a 76-byte spill at 3 levels:
-// 736 -> 714 us at 21600 x 72; the FP32 ice group holds 127 registers anyway).
-template <class R, int G, int L> constexpr int sed_waves_per_simd() {
-  return (G == SED_ICE && sizeof(R) == 8 && L <= 3) ? 3 : (L <= 5 ? 2 : 1);
-}
-
 // Workgroups go to the 8 XCDs round-robin, and each XCD has its own L2.  A
 // block's lanes read a few columns of many levels, so neighbouring blocks share
 // every cache line they touch: give each XCD a contiguous range of blocks, so
@@ -687,29 +710,33 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
   return x * q + (x < r ? x : r) + n;
 }
 
-// Elements of the ice-sedimentation record table (4 quantities per cell).
-constexpr int kIceSedTableLen = densize * rimsize * isize * 4;
-
 // Copies the ice-sedimentation records into the block's LDS copy (16-byte
 // chunks), then waits for the whole block.
-template <class R>
+template <class R, int LEN>
 __device__ __forceinline__ void stage_ice_table(const R* __restrict__ g, R* s) {
-  constexpr int n = kIceSedTableLen * (int)sizeof(R) / 16;
-  static_assert(kIceSedTableLen * sizeof(R) % 16 == 0, "16-byte chunks");
+  constexpr int n = LEN * (int)sizeof(R) / 16;
+  static_assert(LEN * sizeof(R) % 16 == 0, "16-byte chunks");
   for (int c = threadIdx.x; c < n; c += blockDim.x)
     reinterpret_cast<uint4*>(s)[c] = reinterpret_cast<const uint4*>(g)[c];
   __syncthreads();
 }
 
-// The ice group interpolates the ice table from an LDS copy of its records
-// (32 KB in FP64, 16 KB in FP32), staged once per block, rather than through
-// the L1/L2 caches: 16 gathers per level and substep become LDS reads (ice
-// sedimentation 3-13 % faster at 21600 x 72 and 16384 x 128).
+// The ice table's records in the form the ice group of L levels per lane reads.
+template <class R, int L>
+__device__ __forceinline__ const R* ice_lds_source(const KParams& p) {
+  return ice_rec_dif<R, L>() ? TabOf<R>::get(p.tab).ice_sed_d : TabOf<R>::get(p.tab).ice_sed;
+}
+
+// The ice group interpolates the ice table from an LDS copy of its records,
+// staged once per block, rather than through the L1/L2 caches: 16 gathers per
+// level and substep become LDS reads (ice sedimentation 3-13 % faster at
+// 21600 x 72 and 16384 x 128 with the plain records).
 template <class R, int LPC, int L, int WPB, int G>
 __global__ __launch_bounds__(64 * WPB, (sed_waves_per_simd<R, G, L>())) void p3_sed_kernel(const KParams p) {
   if constexpr (G == SED_ICE) {
-    __shared__ __align__(16) R s_tab[kIceSedTableLen];
-    stage_ice_table<R>(TabOf<R>::get(p.tab).ice_sed, s_tab);
+    static_assert(WPB == 4 || !ice_rec_dif<R, L>(), "the LDS budget assumes blocks of 4 wavefronts");
+    __shared__ __align__(16) R s_tab[ice_lds_len<R, L>()];
+    stage_ice_table<R, ice_lds_len<R, L>()>(ice_lds_source<R, L>(p), s_tab);
     sed_block<R, LPC, L, WPB, G>(p, xcd_block(blockIdx.x, gridDim.x), s_tab);
   } else {
     sed_block<R, LPC, L, WPB, G>(p, xcd_block(blockIdx.x, gridDim.x), nullptr);

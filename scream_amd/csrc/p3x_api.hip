@@ -100,9 +100,10 @@ constexpr size_t ice_p2 = revap + p3c::RAIN_TABLE_LEN;
 constexpr size_t ice_sed = ice_p2 + kCells * 8;
 constexpr size_t ice_p3 = ice_sed + kCells * 4;
 constexpr size_t rain_v = ice_p3 + kCells * 4;
-constexpr size_t total = rain_v + (size_t)p3c::RAIN_TABLE_LEN * 2;
-static_assert(revap % 4 == 0 && ice_p2 % 4 == 0 && ice_sed % 4 == 0 && ice_p3 % 4 == 0 && rain_v % 4 == 0,
-              "records must stay 16-byte aligned");
+constexpr size_t ice_sed_d = rain_v + (size_t)p3c::RAIN_TABLE_LEN * 2;
+constexpr size_t total = ice_sed_d + kCells * 8;
+static_assert(revap % 4 == 0 && ice_p2 % 4 == 0 && ice_sed % 4 == 0 && ice_p3 % 4 == 0 && rain_v % 4 == 0 &&
+              ice_sed_d % 4 == 0, "records must stay 16-byte aligned");
 }  // namespace TableLayout
 
 template <class T>
@@ -112,6 +113,7 @@ p3d::TableSet<T> table_set(const T* base) {
   s.revap = base + TableLayout::revap;
   s.ice_p2 = base + TableLayout::ice_p2;
   s.ice_sed = base + TableLayout::ice_sed;
+  s.ice_sed_d = base + TableLayout::ice_sed_d;
   s.ice_p3 = base + TableLayout::ice_p3;
   s.rain_v = base + TableLayout::rain_v;
   return s;
@@ -131,6 +133,14 @@ std::vector<T> pack_tables(const p3x_table_data* t) {
     const double* r = t->ice_table + cell * p3c::ice_table_size;
     for (int q = 0; q < 7; ++q) v[ice_p2 + cell * 8 + q] = (T)r[p3d::kIceP2Quant[q]];
     for (int q = 0; q < 4; ++q) v[ice_sed + cell * 4 + q] = (T)r[p3d::kIceSedQuant[q]];
+    // the differences to the next size cell, in T as the kernels would form them
+    const bool last = cell % p3c::isize == (size_t)p3c::isize - 1;
+    for (int q = 0; q < 4; ++q) {
+      const T a = (T)r[p3d::kIceSedQuant[q]];
+      const T b = last ? a : (T)r[p3c::ice_table_size + p3d::kIceSedQuant[q]];
+      v[ice_sed_d + cell * 8 + q] = a;
+      v[ice_sed_d + cell * 8 + 4 + q] = (T)(b - a);
+    }
     for (int q = 0; q < 4; ++q) v[ice_p3 + cell * 4 + q] = (T)r[p3d::kIceP3Quant[q]];
   }
   for (size_t j = 0; j < (size_t)p3c::RAIN_TABLE_LEN; ++j) {
