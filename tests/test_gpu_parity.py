@@ -110,6 +110,38 @@ def test_fp64_nonzero_history_inputs(ctx, oracle_tables):
     _check_fp64(got, gdbg, want, wdbg, "history in/out")
 
 
+def _low_ice(ncol, nlev, seed):
+    """Columns whose ice starts at a random level in the lower part of the column (the
+    ice above it removed), so that the ice runs cover a short window: at 72 levels on 16
+    lanes they hold 1 or 2 levels per lane inside a kernel built for 5."""
+    inp = make_columns(ncol, nlev, seed=seed, dry_every=7, evap_every=9)
+    top = np.random.default_rng(seed).integers(int(0.6 * nlev), int(0.86 * nlev), size=ncol)
+    above = np.arange(nlev)[:, None] < top[None, :]
+    for k in ("qi", "ni", "qm", "bm"):
+        inp[k] = np.where(above, 0.0, inp[k])
+    return inp
+
+
+def test_fp64_short_ice_windows(ctx, oracle_tables):
+    """Ice runs on fewer levels per lane than their kernel stages its table for."""
+    inp = _low_ice(160, 72, seed=31)
+    want, wdbg, nerr = oracle.run_oracle(inp, oracle_tables, 600.0, True, False, math=MATH)
+    assert nerr == 0
+    assert (wdbg["sed_substeps"][2] > 0).sum() >= 40, "too few columns sediment ice"
+    got, gdbg, _ = run_gpu(ctx, inp, 600.0, True, False)
+    _check_fp64(got, gdbg, want, wdbg, "short ice windows 160x72")
+
+
+def test_mixed_short_ice_windows(ctx, oracle_tables):
+    """The same short ice windows in the mixed form, within its stated error."""
+    inp = _low_ice(160, 72, seed=32)
+    want, wdbg, _ = oracle.run_oracle(inp, oracle_tables, 600.0, True, False, math=MATH)
+    got, gdbg, _ = run_gpu(ctx, inp, 600.0, True, False, precision=p3x.P3X_MIXED)
+    errs = _normalized_errors(got, want)
+    assert max(errs.values()) <= MIXED_BOUND, errs
+    np.testing.assert_array_equal(gdbg["col_status"], wdbg["col_status"])
+
+
 @pytest.mark.parametrize("cb,nt,wpe", [(16, 128, 1), (16, 256, 1), (32, 256, 1), (64, 256, 1), (32, 512, 1),
                                        (64, 512, 1), (16, 256, 2), (32, 256, 2), (64, 256, 2)])
 @pytest.mark.parametrize("lpc", [16, 32, 64])
