@@ -743,4 +743,34 @@ __global__ __launch_bounds__(64 * WPB, (sed_waves_per_simd<R, G, L>())) void p3_
   }
 }
 
+// Both species groups in one launch, for shapes at which they run on the same
+// lanes and levels per lane: blocks
+// [0, nb_ice) run the ice group and the blocks after them the liquid group,
+// over the same column blocks.  Workgroups are dispatched in order, so the
+// short liquid blocks take the CUs that the long ice blocks free at the end of
+// the ice work, the longest work first.  Two kernels side by side on two
+// streams interleave their blocks from the start instead, and the liquid
+// blocks then delay the ice blocks; nor is there a fork or a join between
+// streams.  Each block runs the code of the two-kernel form, so the results
+// are the same bits.  The kernel is compiled for the ice group's wavefronts
+// per SIMD; the shapes it is launched at are those where the liquid group's
+// code also fits that register budget without a spill (in FP64 not at 3
+// levels per lane, where the ice group runs three wavefronts and the liquid
+// group needs 173 VGPRs).
+template <class R, int LPC, int L, int WPB>
+__global__ __launch_bounds__(64 * WPB, (sed_waves_per_simd<R, SED_ICE, L>())) void p3_sed_fused_kernel(
+    const KParams p, int nb_ice) {
+  static_assert(WPB == 4 || !ice_rec_dif<R, L>(), "the LDS budget assumes blocks of 4 wavefronts");
+  __shared__ __align__(16) R s_tab[ice_lds_len<R, L>()];
+  const int b = blockIdx.x;  // uniform over the block: the whole block takes one branch
+  if (b < nb_ice) {
+    stage_ice_table<R, ice_lds_len<R, L>()>(ice_lds_source<R, L>(p), s_tab);
+    sed_block<R, LPC, L, WPB, SED_ICE>(p, xcd_block(b, nb_ice), s_tab);
+  } else {
+    // block b runs on XCD b % 8; the liquid blocks' offset nb_ice only rotates
+    // which XCD receives which contiguous range of column blocks
+    sed_block<R, LPC, L, WPB, SED_LIQ>(p, xcd_block(b - nb_ice, (int)gridDim.x - nb_ice), nullptr);
+  }
+}
+
 }  // namespace p3d

@@ -110,6 +110,17 @@ def test_fp64_nonzero_history_inputs(ctx, oracle_tables):
     _check_fp64(got, gdbg, want, wdbg, "history in/out")
 
 
+@pytest.mark.parametrize("nlev,lpc", [(96, 16), (128, 16), (200, 32), (48, 16)])
+def test_fp64_explicit_lanes_many_levels_per_lane(ctx, oracle_tables, nlev, lpc):
+    """Explicit lanes per column for both groups at 3 to 8 levels per lane (the one-launch
+    sedimentation at 6 and 8, the two-stream form at 3 in FP64)."""
+    inp = make_columns(96, nlev, seed=200 + nlev, dry_every=7, evap_every=9)
+    want, wdbg, nerr = oracle.run_oracle(inp, oracle_tables, 600.0, True, False, math=MATH)
+    assert nerr == 0
+    got, gdbg, _ = run_gpu(ctx, inp, 600.0, True, False, sed_lanes_per_column=lpc)
+    _check_fp64(got, gdbg, want, wdbg, f"96x{nlev} lanes {lpc}")
+
+
 def _low_ice(ncol, nlev, seed):
     """Columns whose ice starts at a random level in the lower part of the column (the
     ice above it removed), so that the ice runs cover a short window: at 72 levels on 16
