@@ -562,7 +562,8 @@ def summarize_pmc(fetch_dir, write_dir, out_path=None):
     scale_w = COPY_PROBE_BYTES / copy_w  # bytes per counter KiB, writes
     kern = {}
     # part 1 is p3_part1_lvl_kernel (one thread per level and column) or p3_part1_kernel;
-    # sedimentation is two kernels (the ice group and the liquid group), summed
+    # sedimentation is one launch of both species groups (p3_sed_fused_kernel) or two
+    # kernels (the ice group and the liquid group), summed
     for key, name in (("p3_part1", "p3_part1_kernel"), ("p3_part2", "p3_part2_kernel"),
                       ("p3_sed", "p3_sed_kernel"), ("p3_part3", "p3_part3_kernel")):
         f = pick(fetch, key)
@@ -648,7 +649,7 @@ def main():
         # algorithmic bytes per step over its event-timed duration, and its measured HBM bytes
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "p3_sed_kernel (ice group, liquid group) + p3_part3_lvl_kernel"},
+                     "kernel": "p3_sed_fused_kernel (ice and liquid groups in one launch) + p3_part3_lvl_kernel"},
     }
     del w
     if not args.no_column_major:
